@@ -1,0 +1,94 @@
+// Host-side internals of the algofft_hip library: error state, the kernel
+// registry, and the context / buffer / stream / plan objects behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/algofft_hip.h"
+#include "kernels.hpp"
+
+namespace afft {
+
+// ---- errors -------------------------------------------------------------------
+int set_error(int status, const std::string& msg);
+void clear_error();
+
+#define AFFT_HIP(expr)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return ::afft::set_error(AFFT_ERR_DEVICE, std::string(#expr) + ": " +            \
+                                                          hipGetErrorString(e_));            \
+    } while (0)
+
+// ---- kernel registry ------------------------------------------------------------
+const KernelInfo* find_kernel(int prec, int n, int var, int dir);
+int max_single_pass(int prec);        // largest power of two held in LDS
+
+// ---- objects ----------------------------------------------------------------
+struct Context {
+    int device = 0;
+    hipDeviceProp_t props{};
+};
+
+struct Buffer {
+    Context* ctx = nullptr;
+    int64_t len = 0;
+    int precision = 0;
+    void* ptr = nullptr;
+};
+
+struct Stream {
+    Context* ctx = nullptr;
+    hipStream_t s = nullptr;
+};
+
+// Where a step reads or writes.
+enum Role { ROLE_SRC = 0, ROLE_DST = 1, ROLE_TMP0 = 2, ROLE_TMP1 = 3, ROLE_COUNT = 4 };
+
+// One kernel launch of a plan.  Pointers inside `args` are filled at exec
+// time from the roles; `in_off`/`out_off` are byte offsets from the role base.
+struct Step {
+    const void* fn = nullptr;     // __global__ entry
+    dim3 grid, block;
+    int lds_bytes = 0;            // dynamic LDS (0: static only)
+    int in_role = ROLE_SRC, out_role = ROLE_DST;
+    int64_t in_off = 0, out_off = 0;
+    std::vector<unsigned char> arg;   // kernel argument block (a POD struct)
+    int in_field = 0, out_field = 0;  // byte offsets of the in/out pointers in arg
+    std::string name;
+};
+
+struct Plan {
+    Context* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    afft_plan_desc desc{};
+    int64_t n = 0;                  // points of one transform (product of dims)
+    int64_t in_elems[2] = {0, 0};   // [fwd, inv] input length (elements of that side)
+    int64_t out_elems[2] = {0, 0};
+    int in_esize[2] = {0, 0};       // bytes per element of each side
+    int out_esize[2] = {0, 0};
+    std::vector<Step> steps[2];     // [0] forward, [1] inverse
+    std::vector<void*> owned;       // device tables
+    size_t tmp_bytes[2] = {0, 0};
+    void* tmp[2] = {nullptr, nullptr};
+    void* stage_in = nullptr;
+    void* stage_out = nullptr;
+    size_t stage_in_bytes = 0, stage_out_bytes = 0;
+    std::string algo;
+    std::mutex mu;
+    ~Plan();
+};
+
+int build_plan(Plan& p);
+int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s);
+
+// FP64 roots of unity W_L^m = exp(-2πi m / L), exact integer reduction of m.
+void root_of_unity(int64_t m, int64_t L, double* re, double* im);
+
+}  // namespace afft

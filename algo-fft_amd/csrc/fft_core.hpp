@@ -1,0 +1,124 @@
+// Compile-time Stockham core: one transform of N = R0*R1*...*R(P-1) points is
+// held by TT = N/E threads, E points per thread in registers.  Thread t owns
+// points n = t + TT*e (e < E) before the first pass and after the last pass
+// ("cyclic" layout), so the first loads and last stores of consecutive threads
+// touch consecutive elements.  Between passes the points go through LDS.
+//
+// Pass p (radix R, Ns = R0*...*R(p-1)), butterflies j = t + TT*b (b < E/R):
+//   in  : x[j + r*N/R], r < R             (register v[b + r*E/R])
+//   tw  : x *= W_{Ns*R}^{(j mod Ns) * r}  (table, forward sign; conj for DIR=+1)
+//   DFT_R in registers
+//   out : y[(j/Ns)*Ns*R + (j mod Ns) + r*Ns]
+// which is the self-sorting Stockham recursion: natural order in and out.
+// The reference's radix-2 Stockham (internal/kernels/stockham.go:21-101)
+// is the R=2 special case of the same recursion.
+#pragma once
+#include <utility>
+
+#include "fft_common.hpp"
+
+namespace afft {
+
+template <int... Rs>
+struct Sched {
+    static constexpr int P = sizeof...(Rs);
+    static constexpr int rad[P] = {Rs...};
+    static constexpr int N = (1 * ... * Rs);
+    static constexpr int ns(int p) {
+        int s = 1;
+        for (int i = 0; i < p; ++i) s *= rad[i];
+        return s;
+    }
+    // offset (in entries) of pass p's twiddle block; pass 0 has none
+    static constexpr int twoff(int p) {
+        int o = 0;
+        for (int i = 1; i < p; ++i) o += ns(i) * (rad[i] - 1);
+        return o;
+    }
+    static constexpr int twsize() { return twoff(P); }
+    static constexpr int maxrad() {
+        int m = 0;
+        for (int i = 0; i < P; ++i) m = rad[i] > m ? rad[i] : m;
+        return m;
+    }
+};
+
+// LDS addressing for transform-major ("row") blocks: transform c occupies a
+// padded slab; one pad slot every 2^PADSH points spreads the strided writes of
+// the early passes over all banks.
+template <int N, int PADSH>
+struct LdsRow {
+    static constexpr int padded(int n) { return PADSH > 0 ? n + (n >> PADSH) : n; }
+    static constexpr int SLAB = padded(N) + (PADSH > 0 ? 1 : 0);
+    int base;
+    __device__ __forceinline__ LdsRow(int c) : base(c * SLAB) {}
+    __device__ __forceinline__ int operator()(int n) const {
+        return base + (PADSH > 0 ? n + (n >> PADSH) : n);
+    }
+};
+
+// LDS addressing for column-interleaved blocks: point n of transform c at
+// n*TPB + c, so lanes that differ in c hit consecutive banks.
+template <int TPB>
+struct LdsCol {
+    int c;
+    __device__ __forceinline__ LdsCol(int c_) : c(c_) {}
+    __device__ __forceinline__ int operator()(int n) const { return n * TPB + c; }
+};
+
+template <typename T, class S, int E, int DIR, int p, class L>
+__device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
+                                          const L& li) {
+    constexpr int N = S::N;
+    constexpr int TT = N / E;
+    constexpr int R = S::rad[p];
+    constexpr int NS = S::ns(p);
+    constexpr int B = E / R;
+    static_assert(E % R == 0, "every radix must divide the points per thread");
+    if constexpr (p > 0) {
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b + r * B] = lds[li(t + TT * b + r * (N / R))];
+    }
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = t + TT * b;
+            const int k = j % NS;
+            const cx<T>* __restrict__ w = tw + S::twoff(p) + k * (R - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], w[r - 1]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) dft<R, DIR, B>(v + b);
+    if constexpr (p < S::P - 1) {
+        if constexpr (p > 0) __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = t + TT * b;
+            const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds[li(base + r * NS)] = v[b + r * B];
+        }
+    }
+}
+
+template <typename T, class S, int E, int DIR, class L, int... Ps>
+__device__ __forceinline__ void core_all(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
+                                         const L& li, std::integer_sequence<int, Ps...>) {
+    (core_pass<T, S, E, DIR, Ps>(v, lds, t, tw, li), ...);
+}
+
+// Full transform.  PRESYNC adds a barrier before the first LDS write, needed
+// when the LDS slab was read earlier in the same kernel.
+template <typename T, class S, int E, int DIR, bool PRESYNC = false, class L>
+__device__ __forceinline__ void core(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
+                                     const L& li) {
+    if constexpr (PRESYNC && S::P > 1) __syncthreads();
+    core_all<T, S, E, DIR>(v, lds, t, tw, li, std::make_integer_sequence<int, S::P>{});
+}
+
+}  // namespace afft

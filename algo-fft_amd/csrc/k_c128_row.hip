@@ -1,0 +1,5 @@
+// Power-of-two c128 kernels, variant VAR_ROW.
+#include "inst.hpp"
+
+AFFT_POW2_SMALL(double, ::afft::VAR_ROW)
+

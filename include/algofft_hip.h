@@ -1,0 +1,174 @@
+/*
+ * algofft_hip — C ABI of the MI355X (gfx950) FFT backend for algo-fft.
+ *
+ * This is the boundary a cgo binding of algo-fft's `gpu` package binds to.
+ * Each entry point names the Go interface method it backs
+ * (gpu/backend.go, gpu/plan.go, gpu/types.go, gpu/errors.go in
+ * github.com/cwbudde/algo-fft).  Plain pointers, sizes and integer status
+ * codes only: no C++ or framework types cross this line.
+ *
+ * Data layout: complex values are interleaved (re, im) exactly like Go's
+ * complex64/complex128 slices; real values are float/double arrays.
+ * Element counts are in complex elements unless stated otherwise.
+ *
+ * Threading: every object may be used from one thread at a time; distinct
+ * plans may run concurrently on distinct streams.  The last error message is
+ * thread-local.
+ */
+#ifndef ALGOFFT_HIP_H
+#define ALGOFFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: 1:1 with the Go sentinel errors ---------------------- */
+typedef enum {
+    AFFT_OK = 0,
+    AFFT_ERR_NO_BACKEND = 1,          /* gpu/errors.go:7   ErrNoBackend */
+    AFFT_ERR_BACKEND_UNAVAILABLE = 2, /* gpu/errors.go:11  ErrBackendUnavailable */
+    AFFT_ERR_NOT_IMPLEMENTED = 3,     /* gpu/errors.go:14  ErrNotImplemented (also wrong dynamic type) */
+    AFFT_ERR_INVALID_LENGTH = 4,      /* gpu/errors.go:17  ErrInvalidLength */
+    AFFT_ERR_NIL_SLICE = 5,           /* gpu/errors.go:20  ErrNilSlice */
+    AFFT_ERR_LENGTH_MISMATCH = 6,     /* gpu/errors.go:23  ErrLengthMismatch */
+    AFFT_ERR_INVALID_STRIDE = 7,      /* errors.go:21      ErrInvalidStride */
+    AFFT_ERR_INVALID_SPECTRUM = 8,    /* errors.go:25      ErrInvalidSpectrum */
+    AFFT_ERR_DEVICE = 9,              /* HIP runtime failure (allocation, launch, copy) */
+    AFFT_ERR_INVALID_ARGUMENT = 10    /* bad handle or descriptor field */
+} afft_status;
+
+/* gpu/types.go:10-13 PrecisionKind */
+typedef enum { AFFT_COMPLEX64 = 0, AFFT_COMPLEX128 = 1 } afft_precision;
+
+typedef enum { AFFT_FORWARD = -1, AFFT_INVERSE = 1 } afft_direction;
+
+/* Transform kinds of the extended plan descriptor. */
+typedef enum { AFFT_C2C = 0, AFFT_R2C = 1, AFFT_C2R = 2 } afft_kind;
+
+/* Execution forms selectable per plan (afft_plan_desc.flags). */
+enum {
+    AFFT_FLAG_INPLACE = 1 << 0,   /* gpu/types.go:40 PlanOptions.InPlace */
+    AFFT_FLAG_MFMA = 1 << 1,      /* radix-16 stages on matrix cores (hi/lo split) */
+    AFFT_FLAG_NORMALIZE = 1 << 2, /* forward scaled by 1/N (PlanRealT.ForwardNormalized) */
+    AFFT_FLAG_UNITARY = 1 << 3,   /* forward scaled by 1/sqrt(N) (ForwardUnitary) */
+    AFFT_FLAG_REF_CHIRP = 1 << 4  /* Bluestein chirp without the k^2 mod 2N reduction */
+};
+
+/* gpu/types.go:16-22 DeviceInfo (strings are NUL-terminated, truncated). */
+typedef struct {
+    char name[128];
+    char vendor[64];
+    char driver[64];
+    int64_t memory_mb;
+    char compute_cap[64];
+} afft_device_info;
+
+/* gpu/types.go:25-29 BackendInfo */
+typedef struct {
+    char name[64];
+    char version[64];
+    char description[256];
+} afft_backend_info;
+
+/* gpu/types.go:33-42 PlanOptions */
+typedef struct {
+    int device_index;
+    int stream_count;
+    int in_place;
+} afft_plan_options;
+
+/* Extended plan descriptor for the surface the Go `gpu` package lacks today
+ * but the CPU Plan API has (plan_options.go:46-76: Batch, Stride; plus the
+ * real and N-D plans).  dims[0..rank-1] are row-major, last fastest.
+ * For C2C, strides/distances are in complex elements; 0 means contiguous
+ * defaults (stride 1, distance = product of dims).  For R2C/C2R the real side
+ * uses real elements and the complex side complex elements; defaults are
+ * the packed layouts N and N/2+1 on the last dimension. */
+typedef struct {
+    int kind;            /* afft_kind */
+    int precision;       /* afft_precision */
+    int rank;            /* 1..8 */
+    int64_t dims[8];
+    int64_t batch;       /* >= 1 */
+    int64_t istride, idist, ostride, odist;
+    int flags;
+} afft_plan_desc;
+
+typedef struct afft_context_s* afft_context;
+typedef struct afft_buffer_s* afft_buffer;
+typedef struct afft_stream_s* afft_stream;
+typedef struct afft_plan_s* afft_plan;
+
+/* ---- errors -------------------------------------------------------------- */
+/* Message of the last failing call on this thread ("" if none). */
+const char* afft_last_error(void);
+/* Fixed text of a status code. */
+const char* afft_status_string(int status);
+
+/* ---- Backend (gpu/backend.go:7-12) --------------------------------------- */
+int afft_backend_get_info(afft_backend_info* out);          /* Backend.Info */
+int afft_backend_available(void);                            /* Backend.Available: 1 or 0 */
+int afft_device_count(int* count);                           /* Backend.Devices (count) */
+int afft_device_get_info(int index, afft_device_info* out); /* Backend.Devices (entry) */
+
+/* ---- Context (gpu/backend.go:15-24) -------------------------------------- */
+int afft_context_create(int device_index, afft_context* out); /* Backend.NewContext */
+int afft_context_device(afft_context ctx, afft_device_info* out); /* Context.Device */
+int afft_context_destroy(afft_context ctx);                   /* Context.Close */
+
+/* ---- Buffer (gpu/backend.go:27-35; Context.NewBuffer :18) ---------------- */
+int afft_buffer_create(afft_context ctx, int64_t elem_count, int precision, afft_buffer* out);
+int64_t afft_buffer_len(afft_buffer buf);                       /* Buffer.Len */
+int afft_buffer_precision(afft_buffer buf);                     /* Buffer.Precision */
+/* Buffer.Upload / Buffer.Download: host_len is the host slice length in
+ * complex elements; it must be >= Len (ErrLengthMismatch otherwise). */
+int afft_buffer_upload(afft_buffer buf, const void* host, int64_t host_len);
+int afft_buffer_download(afft_buffer buf, void* host, int64_t host_len);
+void* afft_buffer_device_ptr(afft_buffer buf);
+int afft_buffer_destroy(afft_buffer buf);                       /* Buffer.Close */
+
+/* ---- Stream (gpu/backend.go:38-41; Context.NewStream :20) ---------------- */
+int afft_stream_create(afft_context ctx, afft_stream* out);
+int afft_stream_synchronize(afft_stream s);                     /* Stream.Synchronize */
+int afft_stream_destroy(afft_stream s);                         /* Stream.Close */
+/* Underlying hipStream_t, as an opaque pointer. */
+void* afft_stream_native(afft_stream s);
+
+/* ---- PlanImpl (gpu/backend.go:45-51; Context.NewFFTPlan :22) ------------- */
+/* Context.NewFFTPlan(n, precision, opts): one 1-D C2C transform of n points. */
+int afft_plan_create(afft_context ctx, int64_t n, int precision, const afft_plan_options* opts,
+                     afft_plan* out);
+/* Extended constructor: batch/stride (Plan.ForwardBatch, plan.go:504;
+ * Plan.ForwardStrided, plan_strided.go:16), real (NewPlanReal32/64,
+ * plan_real_constructors.go:8-43) and N-D (NewPlan2D/3D/ND, plan_2d.go:44,
+ * plan_3d.go:42, plan_nd.go:43). */
+int afft_plan_create_desc(afft_context ctx, const afft_plan_desc* desc, afft_plan* out);
+int64_t afft_plan_len(afft_plan p);           /* PlanImpl.Len: points of one transform */
+int afft_plan_precision(afft_plan p);         /* PlanImpl.Precision */
+/* Required host/device lengths of the input and output side, in elements
+ * of that side's type (complex, or real for the real side of R2C/C2R). */
+int afft_plan_io_len(afft_plan p, int direction, int64_t* in_len, int64_t* out_len);
+/* PlanImpl.Forward / Inverse with host slices (the Go []complex64/128 case):
+ * stages through device memory on the plan's stream and synchronises.
+ * dst may equal src.  Lengths are host slice lengths (elements). */
+int afft_plan_forward(afft_plan p, void* dst, int64_t dst_len, const void* src, int64_t src_len);
+int afft_plan_inverse(afft_plan p, void* dst, int64_t dst_len, const void* src, int64_t src_len);
+/* PlanImpl.Forward / Inverse with device-resident data (the Buffer case):
+ * enqueues on `stream` (hipStream_t as void*, NULL = the plan's stream)
+ * and returns without synchronising. */
+int afft_plan_exec_device(afft_plan p, int direction, void* dst_dev, const void* src_dev, void* stream);
+int afft_plan_exec_buffers(afft_plan p, int direction, afft_buffer dst, afft_buffer src, afft_stream s);
+/* Algorithm string recorded in wisdom (cf. Plan.Algorithm, plan.go:157). */
+int afft_plan_describe(afft_plan p, char* buf, int cap);
+/* Bytes of device workspace the plan holds. */
+int64_t afft_plan_workspace_bytes(afft_plan p);
+int afft_plan_destroy(afft_plan p);           /* PlanImpl.Close */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALGOFFT_HIP_H */
