@@ -49,7 +49,11 @@ struct Stream {
 };
 
 // Where a step reads or writes.
-enum Role { ROLE_SRC = 0, ROLE_DST = 1, ROLE_TMP0 = 2, ROLE_TMP1 = 3, ROLE_COUNT = 4 };
+// TMP0/TMP1 are the scratch of the four-step, Bluestein and gather paths;
+// WORK holds the compact complex array of an N-D inverse real transform and
+// WORK2 the half-length spectra of real rows without a fused kernel.
+enum Role { ROLE_SRC = 0, ROLE_DST = 1, ROLE_TMP0 = 2, ROLE_TMP1 = 3, ROLE_WORK = 4, ROLE_WORK2 = 5, ROLE_COUNT = 6 };
+constexpr int NTMP = ROLE_COUNT - 2;
 
 // One kernel launch of a plan.  Pointers inside `args` are filled at exec
 // time from the roles; `in_off`/`out_off` are byte offsets from the role base.
@@ -75,8 +79,8 @@ struct Plan {
     int out_esize[2] = {0, 0};
     std::vector<Step> steps[2];     // [0] forward, [1] inverse
     std::vector<void*> owned;       // device tables
-    size_t tmp_bytes[2] = {0, 0};
-    void* tmp[2] = {nullptr, nullptr};
+    size_t tmp_bytes[NTMP] = {0, 0, 0, 0};
+    void* tmp[NTMP] = {nullptr, nullptr, nullptr, nullptr};
     void* stage_in = nullptr;
     void* stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;

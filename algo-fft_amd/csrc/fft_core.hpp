@@ -49,7 +49,7 @@ struct Sched {
 template <int N, int PADSH>
 struct LdsRow {
     static constexpr int padded(int n) { return PADSH > 0 ? n + (n >> PADSH) : n; }
-    static constexpr int SLAB = padded(N) + (PADSH > 0 ? 1 : 0);
+    static constexpr int SLAB = padded(N) + 1;   // one spare slot: point N for the real kernels
     int base;
     __device__ __forceinline__ LdsRow(int c) : base(c * SLAB) {}
     __device__ __forceinline__ int operator()(int n) const {

@@ -12,7 +12,7 @@ namespace afft {
 template <typename T, int N, int E, int VAR>
 constexpr int tpb_for() {
     constexpr int TT = N / E;
-    if constexpr (VAR == VAR_ROW) {
+    if constexpr (VAR == VAR_ROW || VAR == VAR_BLUE || VAR == VAR_REAL) {
         return TT >= 256 ? 1 : 256 / TT;
     } else {
         int t = 16;

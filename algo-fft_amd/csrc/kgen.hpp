@@ -1,0 +1,122 @@
+// Runtime-size Stockham kernel for 2/3/5/7-smooth lengths (the mixed-radix
+// path; the reference's mixedradix.go:45-358 does the same recursion
+// depth-first on the CPU and sends 7-smooth sizes to Bluestein instead).
+//
+// One block holds G = 256 / TPT transforms, TPT threads each.  Every pass
+// reads its butterfly inputs from one LDS buffer (global memory for the first
+// pass), applies the per-pass twiddles, runs the radix-R codelet and writes
+// to the other LDS buffer (global memory for the last pass, whose output
+// index j + r*Ns is already natural order).  One barrier per pass.
+#pragma once
+#include "kernels.hpp"
+
+namespace afft {
+
+constexpr int GEN_THREADS = 256;
+constexpr int GEN_MAXPASS = 16;
+
+struct GArgs {
+    KArgs k;
+    int n;
+    int tpt;                     // threads per transform (power of two)
+    int npass;
+    int rad[GEN_MAXPASS];
+    int ns[GEN_MAXPASS];
+    int twoff[GEN_MAXPASS];
+};
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft_rt(int R, cx<T>* v) {
+    switch (R) {
+    case 2: dft<2, DIR, 1>(v); break;
+    case 3: dft<3, DIR, 1>(v); break;
+    case 4: dft<4, DIR, 1>(v); break;
+    case 5: dft<5, DIR, 1>(v); break;
+    case 7: dft<7, DIR, 1>(v); break;
+    case 8: dft<8, DIR, 1>(v); break;
+    case 16: dft<16, DIR, 1>(v); break;
+    default: break;
+    }
+}
+
+// VAR_ROW / VAR_COL / VAR_4A / VAR_4B as in kernels.hpp.  ROW-like variants
+// lay a transform out contiguously in LDS, COL variants interleave G columns.
+template <typename T, int VAR, int DIR>
+__global__ void __launch_bounds__(GEN_THREADS) k_gen(GArgs g) {
+    extern __shared__ unsigned char smraw[];
+    cx<T>* sm = (cx<T>*)smraw;
+    const KArgs& a = g.k;
+    const int N = g.n;
+    const int TPT = g.tpt;
+    const int G = GEN_THREADS / TPT;
+    const int tid = threadIdx.x;
+    constexpr bool ROWL = (VAR == VAR_ROW || VAR == VAR_4B);
+    int c, t;
+    if constexpr (ROWL) { c = tid / TPT; t = tid % TPT; }
+    else { c = tid % G; t = tid / G; }
+    const long long cg = (long long)blockIdx.x * G + c;
+    const bool act = cg < a.count;
+    auto lidx = [&](int buf, int n) -> int {
+        if constexpr (ROWL) return buf * G * N + c * N + n;
+        else return buf * G * N + n * G + c;
+    };
+    const cx<T>* __restrict__ in = (const cx<T>*)a.in + (long long)blockIdx.y * a.in_bd;
+    cx<T>* __restrict__ out = (cx<T>*)a.out + (long long)blockIdx.y * a.out_bd;
+    const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+    const cx<T>* src_g = (VAR == VAR_ROW || VAR == VAR_4B) ? in + cg * a.in_cd : in + cg;
+    const long long ind = (VAR == VAR_ROW || VAR == VAR_4B) ? 1 : a.in_nd;
+
+    for (int p = 0; p < g.npass; ++p) {
+        const int R = g.rad[p];
+        const int NS = g.ns[p];
+        const int NR = N / R;
+        const bool last = p == g.npass - 1;
+        for (int j = t; j < NR; j += TPT) {
+            cx<T> v[16];
+            for (int r = 0; r < R; ++r) {
+                const int n = j + r * NR;
+                if (p == 0) v[r] = act ? src_g[(long long)n * ind] : mk<T>(0, 0);
+                else v[r] = sm[lidx((p - 1) & 1, n)];
+            }
+            if (NS > 1) {
+                const cx<T>* w = tw + g.twoff[p] + (j % NS) * (R - 1);
+                for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], w[r - 1]);
+            }
+            dft_rt<DIR>(R, v);
+            if (!last) {
+                const int base = (j / NS) * NS * R + (j % NS);
+                for (int r = 0; r < R; ++r) sm[lidx(p & 1, base + r * NS)] = v[r];
+            } else {
+                // output index k = j + r*NS (natural order)
+                for (int r = 0; r < R; ++r) {
+                    const int k = j + r * NS;
+                    cx<T> y = v[r];
+                    if constexpr (VAR == VAR_4A) {
+                        const long long m = (long long)k * (cg + a.tw4_c0);
+                        y = twmul<DIR>(y, tw4<T>(a, m));
+                    }
+                    if (a.scale != 1.0) y = scl(y, (T)a.scale);
+                    if constexpr (VAR == VAR_ROW) {
+                        if (act) out[cg * a.out_cd + k] = y;
+                    } else if constexpr (VAR == VAR_COL || VAR == VAR_4A) {
+                        if (act) out[cg + (long long)k * a.out_nd] = y;
+                    } else {
+                        sm[lidx(p & 1, k)] = y;   // re-mapped below
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (VAR == VAR_4B) {
+        // transposing store: lanes run along the transform index c
+        const int fb = (g.npass - 1) & 1;
+        const int c2 = tid % G, t2 = tid / G;
+        const long long cg2 = (long long)blockIdx.x * G + c2;
+        if (cg2 < a.count) {
+            for (int k = t2; k < N; k += TPT) out[cg2 + (long long)k * a.out_nd] = sm[fb * G * N + c2 * N + k];
+        }
+    }
+}
+
+}  // namespace afft

@@ -12,7 +12,10 @@
 #include <algorithm>
 #include <cstring>
 
+#include <cmath>
+
 #include "afft_internal.hpp"
+#include "kgen.hpp"
 
 namespace afft {
 
@@ -242,13 +245,672 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
     return AFFT_OK;
 }
 
-static int emit_axis(Plan& p, int di, const AxisJob& j, std::string& algo) {
-    if (j.n == 1) {
-        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "length-1 axis");
+// ---- runtime-size mixed radix ------------------------------------------------
+const void* gen_kernel(int prec, int var, int dir);
+const void* copy_kernel(int prec);
+
+// Radix schedule for a 2/3/5/7-smooth n: radix-16 passes, the power-of-two
+// remainder, then 3s, 5s and 7s.  Empty if n has another prime factor.
+static std::vector<int> smooth_schedule(int64_t n) {
+    std::vector<int> r;
+    int64_t m = n;
+    int a = 0;
+    while (m % 2 == 0) { m /= 2; ++a; }
+    std::vector<int> odd;
+    for (int q : {3, 5, 7})
+        while (m % q == 0) { m /= q; odd.push_back(q); }
+    if (m != 1) return {};
+    while (a >= 4) { r.push_back(16); a -= 4; }
+    if (a > 0) r.insert(r.begin(), 1 << a);
+    for (int q : odd) r.push_back(q);
+    return r;
+}
+
+static int gen_cap(int prec) { return prec ? 4096 : 8192; }
+
+static int64_t next_pow2(int64_t n) {
+    int64_t m = 1;
+    while (m < n) m <<= 1;
+    return m;
+}
+
+struct GenCfg {
+    int tpt, g;
+    int lds;
+};
+
+static GenCfg gen_cfg(int64_t n, int prec) {
+    GenCfg c;
+    c.tpt = (int)std::min<int64_t>(256, next_pow2((n + 15) / 16));
+    c.g = GEN_THREADS / c.tpt;
+    c.lds = (int)(2 * c.g * n * (prec ? 16 : 8));
+    return c;
+}
+
+static int gen_args(Plan& p, int64_t n, GArgs& g, int* tpt, int* lds) {
+    std::vector<int> rad = smooth_schedule(n);
+    if (rad.empty() || (int)rad.size() > GEN_MAXPASS) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no radix schedule");
+    KernelInfo fake{};
+    fake.prec = p.desc.precision;
+    // reuse the schedule twiddle generator with the runtime radix list
+    std::vector<double> tw;
+    int64_t ns = rad[0];
+    g.npass = (int)rad.size();
+    g.ns[0] = 1;
+    g.twoff[0] = 0;
+    g.rad[0] = rad[0];
+    for (size_t i = 1; i < rad.size(); ++i) {
+        const int R = rad[i];
+        g.rad[i] = R;
+        g.ns[i] = (int)ns;
+        g.twoff[i] = (int)(tw.size() / 2);
+        for (int64_t kk = 0; kk < ns; ++kk)
+            for (int r = 1; r < R; ++r) {
+                double re, im;
+                root_of_unity(kk * r, ns * R, &re, &im);
+                tw.push_back(re);
+                tw.push_back(im);
+            }
+        ns *= R;
     }
-    if (is_pow2(j.n) && j.n <= max_single_pass(p.desc.precision)) return emit_single(p, di, j, algo);
+    int rc;
+    if (p.desc.precision == 0) {
+        std::vector<float> f(tw.begin(), tw.end());
+        rc = upload_table(p, f.data(), f.size() * 4, &g.k.tw);
+    } else {
+        rc = upload_table(p, tw.data(), tw.size() * 8, &g.k.tw);
+    }
+    if (rc) return rc;
+    GenCfg c = gen_cfg(n, p.desc.precision);
+    g.n = (int)n;
+    g.tpt = c.tpt;
+    *tpt = c.tpt;
+    *lds = c.lds;
+    return AFFT_OK;
+}
+
+static Step make_gen_step(const void* fn, const GArgs& g, int64_t gx, int lds, const std::string& name) {
+    Step s;
+    s.fn = fn;
+    s.grid = dim3((unsigned)gx, 1, 1);
+    s.block = dim3(GEN_THREADS, 1, 1);
+    s.lds_bytes = lds;
+    s.arg.resize(sizeof(GArgs));
+    std::memcpy(s.arg.data(), &g, sizeof(GArgs));
+    s.in_field = (int)(offsetof(GArgs, k) + offsetof(KArgs, in));
+    s.out_field = (int)(offsetof(GArgs, k) + offsetof(KArgs, out));
+    s.name = name;
+    return s;
+}
+
+static int prep_dynamic_lds(const void* fn, int lds) {
+    if (lds > 65536) AFFT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    return AFFT_OK;
+}
+
+static int emit_gen_single(Plan& p, int di, const AxisJob& j, std::string& algo) {
+    const int prec = p.desc.precision;
+    const int dir = di ? 1 : -1;
+    const int es = prec ? 16 : 8;
+    int var;
+    int64_t count = j.count, slices = j.slices, in_bd = j.in_bd, out_bd = j.out_bd;
+    if (j.in_nd == 1 && j.out_nd == 1) var = VAR_ROW;
+    else if (j.in_cd == 1 && j.out_cd == 1) var = VAR_COL;
+    else {
+        var = VAR_COL;
+        if (count != 1) {
+            if (slices != 1) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "layout with no unit stride and two batch axes");
+            slices = count;
+            in_bd = j.in_cd;
+            out_bd = j.out_cd;
+            count = 1;
+        }
+    }
+    GArgs g{};
+    int tpt, lds, rc;
+    if ((rc = gen_args(p, j.n, g, &tpt, &lds))) return rc;
+    const void* fn = gen_kernel(prec, var, dir);
+    if ((rc = prep_dynamic_lds(fn, lds))) return rc;
+    g.k.count = count;
+    g.k.in_cd = j.in_cd;
+    g.k.out_cd = j.out_cd;
+    g.k.in_nd = j.in_nd;
+    g.k.out_nd = j.out_nd;
+    g.k.in_bd = in_bd;
+    g.k.out_bd = out_bd;
+    g.k.scale = j.scale;
+    const int G = GEN_THREADS / tpt;
+    Step s = make_gen_step(fn, g, (count + G - 1) / G, lds, "gen" + std::to_string(j.n));
+    s.in_role = j.in_role;
+    s.out_role = j.out_role;
+    s.in_off = j.in_off;
+    s.out_off = j.out_off;
+    push_sliced(p.steps[di], s, slices, in_bd * es, out_bd * es);
+    if (!algo.empty()) algo += "+";
+    algo += "mixed" + std::to_string(j.n) + (var == VAR_ROW ? "r" : "c");
+    return AFFT_OK;
+}
+
+// Four-step over runtime kernels for smooth n beyond one LDS pass.
+static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
+    const int prec = p.desc.precision;
+    const int dir = di ? 1 : -1;
+    const int es = prec ? 16 : 8;
+    if (j.in_nd != 1 || j.out_nd != 1) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step needs unit stride");
+    const int cap = gen_cap(prec);
+    int64_t n1 = 0;
+    for (int64_t d = 1; d * d <= j.n; ++d)
+        if (j.n % d == 0 && d <= cap && j.n / d <= cap) n1 = d;
+    if (!n1) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no four-step split for n=" + std::to_string(j.n));
+    const int64_t n2 = j.n / n1;
+    const int64_t total = j.count * j.slices;
+    const size_t need = (size_t)(total * j.n * es);
+    if (need > p.tmp_bytes[0]) p.tmp_bytes[0] = need;
+    int rc;
+    GArgs ga{}, gb{};
+    int tpa, ldsa, tpb, ldsb;
+    if ((rc = gen_args(p, n1, ga, &tpa, &ldsa))) return rc;
+    if ((rc = gen_args(p, n2, gb, &tpb, &ldsb))) return rc;
+    if ((rc = split_twiddles(p, prec, j.n, &ga.k.tw4_shift, &ga.k.tw4_lo, &ga.k.tw4_hi))) return rc;
+    const void* fa = gen_kernel(prec, VAR_4A, dir);
+    const void* fb = gen_kernel(prec, VAR_4B, dir);
+    if ((rc = prep_dynamic_lds(fa, ldsa)) || (rc = prep_dynamic_lds(fb, ldsb))) return rc;
+    ga.k.tw4_L = j.n;
+    ga.k.count = n2;
+    ga.k.in_cd = 1; ga.k.in_nd = n2; ga.k.out_cd = 1; ga.k.out_nd = n2;
+    ga.k.out_bd = j.n;
+    ga.k.scale = 1.0;
+    gb.k.count = n1;
+    gb.k.in_cd = n2; gb.k.in_nd = 1; gb.k.in_bd = j.n;
+    gb.k.out_cd = 1; gb.k.out_nd = n1;
+    gb.k.scale = j.scale;
+    const int Ga = GEN_THREADS / tpa, Gb = GEN_THREADS / tpb;
+    for (int64_t y = 0; y < j.slices; ++y) {
+        GArgs a2 = ga, b2 = gb;
+        a2.k.in_bd = j.in_cd;
+        b2.k.out_bd = j.out_cd;
+        Step sa = make_gen_step(fa, a2, (n2 + Ga - 1) / Ga, ldsa, "gen4A");
+        sa.in_role = j.in_role;
+        sa.in_off = j.in_off + y * j.in_bd * es;
+        sa.out_role = ROLE_TMP0;
+        sa.out_off = y * j.count * j.n * es;
+        push_sliced(p.steps[di], sa, j.count, j.in_cd * es, j.n * es);
+        Step sb = make_gen_step(fb, b2, (n1 + Gb - 1) / Gb, ldsb, "gen4B");
+        sb.in_role = ROLE_TMP0;
+        sb.in_off = y * j.count * j.n * es;
+        sb.out_role = j.out_role;
+        sb.out_off = j.out_off + y * j.out_bd * es;
+        push_sliced(p.steps[di], sb, j.count, j.n * es, j.out_cd * es);
+    }
+    if (!algo.empty()) algo += "+";
+    algo += "mixed4step" + std::to_string(n1) + "x" + std::to_string(n2);
+    return AFFT_OK;
+}
+
+// ---- Bluestein ---------------------------------------------------------------
+// In-place iterative radix-2 FFT in long double: builds the Bluestein filter
+// spectrum at plan time.
+static void host_fft_ld(std::vector<long double>& re, std::vector<long double>& im) {
+    const size_t n = re.size();
+    for (size_t i = 1, jj = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; jj & bit; bit >>= 1) jj ^= bit;
+        jj ^= bit;
+        if (i < jj) { std::swap(re[i], re[jj]); std::swap(im[i], im[jj]); }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t half = len / 2, step = n / len;
+        for (size_t k = 0; k < half; ++k) {
+            double wr, wi;
+            root_of_unity((int64_t)(k * step), (int64_t)n, &wr, &wi);
+            // refine the twiddle to long double via the double value (exact
+            // inputs to 1 ulp of double; adequate for a double filter)
+            const long double cr = wr, ci = wi;
+            for (size_t b = 0; b < n; b += len) {
+                const size_t u = b + k, v = b + k + half;
+                const long double xr = re[v] * cr - im[v] * ci, xi = re[v] * ci + im[v] * cr;
+                re[v] = re[u] - xr; im[v] = im[u] - xi;
+                re[u] += xr; im[u] += xi;
+            }
+        }
+    }
+}
+
+struct BlueTables {
+    const void* chirp = nullptr;     // N, forward sign
+    const void* filt[2] = {nullptr, nullptr};  // [fwd, inv], 1/M folded in
+    const void* post[2] = {nullptr, nullptr};  // [fwd, inv]: chirp, conj(chirp)*scale
+};
+
+// chirp[k] = exp(-iπ k²/N); with exact reduction k² mod 2N in integers
+// unless the reference's unreduced formula is requested
+// (internal/kernels/bluestein.go:20-32).
+static int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt) {
+    const bool refchirp = p.desc.flags & AFFT_FLAG_REF_CHIRP;
+    std::vector<double> ch(2 * n);
+    for (int64_t k = 0; k < n; ++k) {
+        if (refchirp) {
+            const double a = -3.14159265358979323846 * (double)(k * k) * (1.0 / (double)n);
+            ch[2 * k] = std::cos(a);
+            ch[2 * k + 1] = std::sin(a);
+        } else {
+            const unsigned __int128 kk = (unsigned __int128)k * (unsigned __int128)k;
+            const int64_t r = (int64_t)(kk % (unsigned __int128)(2 * n));
+            root_of_unity(r, 2 * n, &ch[2 * k], &ch[2 * k + 1]);   // exp(-2πi r/(2N))
+        }
+    }
+    std::vector<double> filt[2];
+    for (int d = 0; d < 2; ++d) {
+        std::vector<long double> re(m, 0.0L), im(m, 0.0L);
+        // forward: b[k] = conj(chirp[k]); inverse: b[k] = chirp[k]; mirrored
+        const long double sgn = d == 0 ? -1.0L : 1.0L;
+        re[0] = ch[0];
+        im[0] = sgn * ch[1];
+        for (int64_t k = 1; k < n; ++k) {
+            re[k] = re[m - k] = ch[2 * k];
+            im[k] = im[m - k] = sgn * ch[2 * k + 1];
+        }
+        host_fft_ld(re, im);
+        filt[d].resize(2 * m);
+        for (int64_t k = 0; k < m; ++k) {
+            filt[d][2 * k] = (double)(re[k] / (long double)m);
+            filt[d][2 * k + 1] = (double)(im[k] / (long double)m);
+        }
+    }
+    std::vector<double> post[2];
+    post[0] = ch;
+    post[1].resize(2 * n);
+    for (int64_t k = 0; k < n; ++k) {
+        post[1][2 * k] = ch[2 * k] * inv_scale;
+        post[1][2 * k + 1] = -ch[2 * k + 1] * inv_scale;
+    }
+    auto up = [&](const std::vector<double>& v, const void** dev) -> int {
+        if (p.desc.precision == 0) {
+            std::vector<float> f(v.begin(), v.end());
+            return upload_table(p, f.data(), f.size() * 4, dev);
+        }
+        return upload_table(p, v.data(), v.size() * 8, dev);
+    };
+    int rc;
+    if ((rc = up(ch, &bt.chirp))) return rc;
+    for (int d = 0; d < 2; ++d) {
+        if ((rc = up(filt[d], &bt.filt[d]))) return rc;
+        if ((rc = up(post[d], &bt.post[d]))) return rc;
+    }
+    return AFFT_OK;
+}
+
+static int emit_bluestein(Plan& p, int di, const AxisJob& j, std::string& algo) {
+    const int prec = p.desc.precision;
+    const int dir = di ? 1 : -1;
+    const int es = prec ? 16 : 8;
+    if (j.in_nd != 1 || j.out_nd != 1) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "Bluestein needs unit stride");
+    const int64_t n = j.n, m = next_pow2(2 * n - 1);
+    BlueTables bt;
+    // the inverse scale rides on the post-multiply table in the chained form
+    int rc = blue_tables(p, n, m, di ? j.scale : 1.0, bt);
+    if (rc) return rc;
+    if (m <= max_single_pass(prec)) {
+        const KernelInfo* k = find_kernel(prec, (int)m, VAR_BLUE, dir);
+        if (!k) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no Bluestein kernel for M=" + std::to_string(m));
+        KArgs dummy{};
+        int rc2 = schedule_twiddles(p, k, &dummy.tw);
+        if (rc2) return rc2;
+        BArgs b{};
+        b.tw = dummy.tw;
+        b.chirp = bt.chirp;
+        b.filt = bt.filt[di];
+        b.count = j.count;
+        b.in_cd = j.in_cd;
+        b.out_cd = j.out_cd;
+        b.nlog = n;
+        b.scale = j.scale;
+        Step s;
+        s.fn = k->fn;
+        s.grid = dim3((unsigned)((j.count + k->tpb - 1) / k->tpb), 1, 1);
+        s.block = dim3((unsigned)k->threads, 1, 1);
+        s.arg.resize(sizeof(BArgs));
+        std::memcpy(s.arg.data(), &b, sizeof(BArgs));
+        s.in_field = (int)offsetof(BArgs, in);
+        s.out_field = (int)offsetof(BArgs, out);
+        s.in_role = j.in_role;
+        s.out_role = j.out_role;
+        s.in_off = j.in_off;
+        s.out_off = j.out_off;
+        s.name = "bluestein" + std::to_string(n);
+        for (int64_t y = 0; y < j.slices; ++y) {
+            Step t = s;
+            t.in_off += y * j.in_bd * es;
+            t.out_off += y * j.out_bd * es;
+            p.steps[di].push_back(t);
+        }
+        if (!algo.empty()) algo += "+";
+        algo += "bluestein" + std::to_string(n) + "m" + std::to_string(m);
+        return AFFT_OK;
+    }
+    // chained form on the four-step kernels of M = m1*m2
+    const int lg = ilog2(m);
+    const int64_t m1 = 1LL << (lg / 2), m2 = m / m1;
+    if (m2 > max_single_pass(prec)) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "Bluestein length too large");
+    const KernelInfo* ka[2] = {find_kernel(prec, (int)m1, VAR_4A, -1), find_kernel(prec, (int)m1, VAR_4A, 1)};
+    const KernelInfo* kb[2] = {find_kernel(prec, (int)m2, VAR_4B, -1), find_kernel(prec, (int)m2, VAR_4B, 1)};
+    if (!ka[0] || !ka[1] || !kb[0] || !kb[1]) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "missing four-step kernel");
+    const int64_t total = j.count * j.slices;
+    const size_t need = (size_t)(total * m * es);
+    for (int t = 0; t < 2; ++t)
+        if (need > p.tmp_bytes[t]) p.tmp_bytes[t] = need;
+    KArgs A[2]{}, B[2]{};
+    for (int d = 0; d < 2; ++d) {
+        if ((rc = schedule_twiddles(p, ka[d], &A[d].tw))) return rc;
+        if ((rc = schedule_twiddles(p, kb[d], &B[d].tw))) return rc;
+        if ((rc = split_twiddles(p, prec, m, &A[d].tw4_shift, &A[d].tw4_lo, &A[d].tw4_hi))) return rc;
+        A[d].tw4_L = m;
+        A[d].count = m2;
+        A[d].in_cd = 1; A[d].in_nd = m2; A[d].out_cd = 1; A[d].out_nd = m2;
+        A[d].out_bd = m;
+        A[d].scale = 1.0;
+        B[d].count = m1;
+        B[d].in_cd = m2; B[d].in_nd = 1; B[d].in_bd = m;
+        B[d].out_cd = 1; B[d].out_nd = m1;
+        B[d].scale = 1.0;
+    }
+    // A1: chirp-multiply + zero-pad fused into the first column pass
+    A[0].fuse = FUSE_CHIRP_IN;
+    A[0].fvec = bt.chirp;
+    A[0].fconj = di;
+    A[0].flen = n;
+    // B1: pointwise filter product fused into the transposing store
+    B[0].fuse = FUSE_MUL_OUT;
+    B[0].fvec = bt.filt[di];
+    B[0].flen = m;
+    B[0].out_bd = m;
+    // B2: post-chirp (and 1/N for the inverse), cropped to n points
+    B[1].fuse = FUSE_CROP_OUT;
+    B[1].fvec = bt.post[di];
+    B[1].flen = n;
+    for (int64_t y = 0; y < j.slices; ++y) {
+        const int64_t tb = y * j.count * m * es;
+        KArgs a1 = A[0];
+        a1.in_bd = j.in_cd;
+        Step s1 = make_step(ka[0], a1, (m2 + ka[0]->tpb - 1) / ka[0]->tpb, 1, "blueA1");
+        s1.in_role = j.in_role; s1.in_off = j.in_off + y * j.in_bd * es;
+        s1.out_role = ROLE_TMP0; s1.out_off = tb;
+        push_sliced(p.steps[di], s1, j.count, j.in_cd * es, m * es);
+        Step s2 = make_step(kb[0], B[0], (m1 + kb[0]->tpb - 1) / kb[0]->tpb, 1, "blueB1");
+        s2.in_role = ROLE_TMP0; s2.in_off = tb;
+        s2.out_role = ROLE_TMP1; s2.out_off = tb;
+        push_sliced(p.steps[di], s2, j.count, m * es, m * es);
+        KArgs a2 = A[1];
+        a2.in_bd = m;
+        Step s3 = make_step(ka[1], a2, (m2 + ka[1]->tpb - 1) / ka[1]->tpb, 1, "blueA2");
+        s3.in_role = ROLE_TMP1; s3.in_off = tb;
+        s3.out_role = ROLE_TMP0; s3.out_off = tb;
+        push_sliced(p.steps[di], s3, j.count, m * es, m * es);
+        KArgs b2 = B[1];
+        b2.out_bd = j.out_cd;
+        Step s4 = make_step(kb[1], b2, (m1 + kb[1]->tpb - 1) / kb[1]->tpb, 1, "blueB2");
+        s4.in_role = ROLE_TMP0; s4.in_off = tb;
+        s4.out_role = j.out_role; s4.out_off = j.out_off + y * j.out_bd * es;
+        push_sliced(p.steps[di], s4, j.count, m * es, j.out_cd * es);
+    }
+    if (!algo.empty()) algo += "+";
+    algo += "bluestein" + std::to_string(n) + "m" + std::to_string(m1) + "x" + std::to_string(m2);
+    return AFFT_OK;
+}
+
+// ---- copies (length-1 axes, gathers) ------------------------------------------
+static Step make_copy(int prec, int64_t lines, int64_t len, int64_t icd, int64_t ind, int64_t ocd, int64_t ond,
+                      double scale) {
+    CopyArgs c{};
+    c.lines = lines;
+    c.len = len;
+    c.in_cd = icd; c.in_nd = ind; c.out_cd = ocd; c.out_nd = ond;
+    c.scale = scale;
+    Step s;
+    s.fn = copy_kernel(prec);
+    const int64_t tot = lines * len;
+    s.grid = dim3((unsigned)((tot + 255) / 256), 1, 1);
+    s.block = dim3(256, 1, 1);
+    s.arg.resize(sizeof(CopyArgs));
+    std::memcpy(s.arg.data(), &c, sizeof(CopyArgs));
+    s.in_field = (int)offsetof(CopyArgs, in);
+    s.out_field = (int)offsetof(CopyArgs, out);
+    s.name = "copy";
+    return s;
+}
+
+static int emit_axis_direct(Plan& p, int di, const AxisJob& j, std::string& algo);
+
+// Paths that need unit-stride rows get strided lines through a gather into
+// TMP1, the row transform in place there, and a scatter back.
+static int emit_via_gather(Plan& p, int di, const AxisJob& j, std::string& algo) {
+    const int prec = p.desc.precision;
+    const int es = prec ? 16 : 8;
+    const int64_t lines = j.count * j.slices;
+    const size_t need = (size_t)(lines * j.n * es);
+    if (need > p.tmp_bytes[1]) p.tmp_bytes[1] = need;
+    for (int64_t y = 0; y < j.slices; ++y) {
+        Step g = make_copy(prec, j.count, j.n, j.in_cd, j.in_nd, j.n, 1, 1.0);
+        g.in_role = j.in_role; g.in_off = j.in_off + y * j.in_bd * es;
+        g.out_role = ROLE_TMP1; g.out_off = y * j.count * j.n * es;
+        p.steps[di].push_back(g);
+    }
+    AxisJob r = j;
+    r.count = lines;
+    r.slices = 1;
+    r.in_cd = r.out_cd = j.n;
+    r.in_nd = r.out_nd = 1;
+    r.in_bd = r.out_bd = 0;
+    r.in_role = r.out_role = ROLE_TMP1;
+    r.in_off = r.out_off = 0;
+    int rc = emit_axis_direct(p, di, r, algo);
+    if (rc) return rc;
+    for (int64_t y = 0; y < j.slices; ++y) {
+        Step s = make_copy(prec, j.count, j.n, j.n, 1, j.out_cd, j.out_nd, 1.0);
+        s.in_role = ROLE_TMP1; s.in_off = y * j.count * j.n * es;
+        s.out_role = j.out_role; s.out_off = j.out_off + y * j.out_bd * es;
+        p.steps[di].push_back(s);
+    }
+    algo += "(gathered)";
+    return AFFT_OK;
+}
+
+static bool needs_rows(const Plan& p, int64_t n) {
+    const int prec = p.desc.precision;
+    if ((n & (n - 1)) == 0) return n > max_single_pass(prec);
+    if (!smooth_schedule(n).empty()) return n > gen_cap(prec);
+    return true;  // Bluestein
+}
+
+static int emit_axis_direct(Plan& p, int di, const AxisJob& j, std::string& algo) {
+    const int prec = p.desc.precision;
+    if (is_pow2(j.n) && j.n <= max_single_pass(prec)) return emit_single(p, di, j, algo);
     if (is_pow2(j.n)) return emit_fourstep(p, di, j, algo);
-    return set_error(AFFT_ERR_NOT_IMPLEMENTED, "size " + std::to_string(j.n) + " not supported yet");
+    if (!smooth_schedule(j.n).empty()) {
+        if (j.n <= gen_cap(prec)) return emit_gen_single(p, di, j, algo);
+        return emit_gen_fourstep(p, di, j, algo);
+    }
+    return emit_bluestein(p, di, j, algo);
+}
+
+static int emit_axis(Plan& p, int di, const AxisJob& j, std::string& algo) {
+    const int prec = p.desc.precision;
+    const int es = prec ? 16 : 8;
+    if (j.n == 1) {
+        for (int64_t y = 0; y < j.slices; ++y) {
+            Step s = make_copy(prec, j.count, 1, j.in_cd, 1, j.out_cd, 1, j.scale);
+            s.in_role = j.in_role; s.in_off = j.in_off + y * j.in_bd * es;
+            s.out_role = j.out_role; s.out_off = j.out_off + y * j.out_bd * es;
+            p.steps[di].push_back(s);
+        }
+        if (!algo.empty()) algo += "+";
+        algo += "copy";
+        return AFFT_OK;
+    }
+    if (needs_rows(p, j.n) && (j.in_nd != 1 || j.out_nd != 1)) return emit_via_gather(p, di, j, algo);
+    return emit_axis_direct(p, di, j, algo);
+}
+
+// Complex transforms over the axes `axes` (processed in the given order) of
+// a row-major array with dims cd[0..rank-1]; batch copies at distances
+// in_dist / out_dist (elements).  The first axis reads `in_role`, every axis
+// writes `out_role`, later axes work in place there.
+static int emit_nd_axes(Plan& p, int di, int rank, const int64_t* cd, const std::vector<int>& axes, int64_t batch,
+                        int in_role, int64_t in_dist, int out_role, int64_t out_dist, std::string& algo) {
+    int64_t total = 1;
+    for (int i = 0; i < rank; ++i) total *= cd[i];
+    bool first = true;
+    for (int ax : axes) {
+        const int64_t len = cd[ax];
+        int64_t inner = 1;
+        for (int i = ax + 1; i < rank; ++i) inner *= cd[i];
+        const int64_t outer = total / (len * inner);
+        if (len == 1 && !(first && in_role != out_role)) continue;   // identity axis
+        AxisJob j{};
+        j.n = len;
+        j.scale = di ? 1.0 / (double)len : 1.0;
+        j.in_role = first ? in_role : out_role;
+        j.out_role = out_role;
+        const int64_t idd = first ? in_dist : out_dist;
+        if (inner == 1) {
+            if (idd == total && out_dist == total) {
+                j.count = outer * batch; j.slices = 1;
+                j.in_cd = len; j.in_nd = 1; j.in_bd = 0;
+                j.out_cd = len; j.out_nd = 1; j.out_bd = 0;
+            } else {
+                j.count = outer; j.slices = batch;
+                j.in_cd = len; j.in_nd = 1; j.in_bd = idd;
+                j.out_cd = len; j.out_nd = 1; j.out_bd = out_dist;
+            }
+            int rc = emit_axis(p, di, j, algo);
+            if (rc) return rc;
+        } else {
+            // columns: count = inner adjacent columns, slices = outer (per batch
+            // copy, or across all copies when they are packed)
+            const int es = p.desc.precision ? 16 : 8;
+            const bool packed = idd == total && out_dist == total;
+            const int64_t nb = packed ? 1 : batch;
+            for (int64_t b = 0; b < nb; ++b) {
+                AxisJob k = j;
+                k.count = inner;
+                k.slices = packed ? outer * batch : outer;
+                k.in_cd = 1; k.in_nd = inner; k.in_bd = len * inner;
+                k.out_cd = 1; k.out_nd = inner; k.out_bd = len * inner;
+                k.in_off = b * idd * es;
+                k.out_off = b * out_dist * es;
+                int rc = emit_axis(p, di, k, algo);
+                if (rc) return rc;
+            }
+        }
+        first = false;
+    }
+    return AFFT_OK;
+}
+
+const void* rpost_kernel(int prec, int dir);
+
+// Real rows of n reals (h = n/2 complex points): `rows` rows, row r of slice y
+// at y*bd + r*rd (complex elements of each side; the real side counts the
+// row as h complex).  R2C when di == 0, C2R when di == 1.
+static int emit_real_rows(Plan& p, int di, int64_t n, int64_t rows, int64_t slices, int in_role, int64_t in_rd,
+                          int64_t in_bd, int out_role, int64_t out_rd, int64_t out_bd, double scale,
+                          std::string& algo) {
+    const int prec = p.desc.precision;
+    const int es = prec ? 16 : 8;
+    const int dir = di ? 1 : -1;
+    const int64_t h = n / 2;
+    // W_N^k, k < h
+    std::vector<double> w(2 * (h + 1));
+    for (int64_t k = 0; k <= h; ++k) root_of_unity(k, n, &w[2 * k], &w[2 * k + 1]);
+    const void* wk;
+    int rc;
+    if (prec == 0) {
+        std::vector<float> f(w.begin(), w.end());
+        rc = upload_table(p, f.data(), f.size() * 4, &wk);
+    } else {
+        rc = upload_table(p, w.data(), w.size() * 8, &wk);
+    }
+    if (rc) return rc;
+    const KernelInfo* k = (h >= 2 && is_pow2(h)) ? find_kernel(prec, (int)h, VAR_REAL, dir) : nullptr;
+    if (k) {
+        RArgs a{};
+        KArgs tmpk{};
+        if ((rc = schedule_twiddles(p, k, &tmpk.tw))) return rc;
+        a.tw = tmpk.tw;
+        a.wk = wk;
+        a.count = rows;
+        a.in_d = in_rd;
+        a.out_d = out_rd;
+        a.scale = di ? scale / (double)h : scale;
+        for (int64_t y = 0; y < slices; ++y) {
+            Step s;
+            s.fn = k->fn;
+            s.grid = dim3((unsigned)((rows + k->tpb - 1) / k->tpb), 1, 1);
+            s.block = dim3((unsigned)k->threads, 1, 1);
+            s.arg.resize(sizeof(RArgs));
+            std::memcpy(s.arg.data(), &a, sizeof(RArgs));
+            s.in_field = (int)offsetof(RArgs, in);
+            s.out_field = (int)offsetof(RArgs, out);
+            s.in_role = in_role;
+            s.out_role = out_role;
+            s.in_off = y * in_bd * es;
+            s.out_off = y * out_bd * es;
+            s.name = di ? "c2r" : "r2c";
+            p.steps[di].push_back(s);
+        }
+        if (!algo.empty()) algo += "+";
+        algo += (di ? "c2r" : "r2c") + std::to_string(n);
+        return AFFT_OK;
+    }
+    // unfused: h-point complex transform as its own steps plus a recombination
+    const size_t need = (size_t)(slices * rows * (h + 1) * es);
+    if (need > p.tmp_bytes[ROLE_WORK2 - 2]) p.tmp_bytes[ROLE_WORK2 - 2] = need;
+    auto post = [&](int irole, int64_t ird, int64_t ibd, int orole, int64_t ord, int64_t obd, double sc) {
+        RPArgs r{};
+        r.wk = wk;
+        r.rows = rows;
+        r.h = h;
+        r.in_d = ird;
+        r.out_d = ord;
+        r.scale = sc;
+        const int64_t per = di ? h : h + 1;
+        for (int64_t y = 0; y < slices; ++y) {
+            Step s;
+            s.fn = rpost_kernel(prec, dir);
+            s.grid = dim3((unsigned)((rows * per + 255) / 256), 1, 1);
+            s.block = dim3(256, 1, 1);
+            s.arg.resize(sizeof(RPArgs));
+            std::memcpy(s.arg.data(), &r, sizeof(RPArgs));
+            s.in_field = (int)offsetof(RPArgs, in);
+            s.out_field = (int)offsetof(RPArgs, out);
+            s.in_role = irole;
+            s.out_role = orole;
+            s.in_off = y * ibd * es;
+            s.out_off = y * obd * es;
+            s.name = "rpost";
+            p.steps[di].push_back(s);
+        }
+    };
+    AxisJob j{};
+    j.n = h;
+    j.count = rows;
+    j.slices = slices;
+    j.in_nd = j.out_nd = 1;
+    if (!di) {
+        j.in_role = in_role; j.in_cd = in_rd; j.in_bd = in_bd;
+        j.out_role = ROLE_WORK2; j.out_cd = h; j.out_bd = rows * h;
+        j.scale = 1.0;
+        if ((rc = emit_axis(p, di, j, algo))) return rc;
+        post(ROLE_WORK2, h, rows * h, out_role, out_rd, out_bd, scale);
+    } else {
+        post(in_role, in_rd, in_bd, ROLE_WORK2, h, rows * h, 1.0);
+        j.in_role = ROLE_WORK2; j.in_cd = h; j.in_bd = rows * h;
+        j.out_role = out_role; j.out_cd = out_rd; j.out_bd = out_bd;
+        j.scale = scale / (double)h;
+        if ((rc = emit_axis(p, di, j, algo))) return rc;
+    }
+    algo += di ? "(c2r)" : "(r2c)";
+    return AFFT_OK;
 }
 
 int build_plan(Plan& p) {
@@ -267,91 +929,125 @@ int build_plan(Plan& p) {
         return set_error(AFFT_ERR_INVALID_STRIDE, "negative stride or distance");
     p.n = n;
     const int es = d.precision ? 16 : 8;
-    if (d.kind != AFFT_C2C) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "real transforms not built yet");
+    const int rs = es / 2;
+    const bool real = d.kind == AFFT_R2C || d.kind == AFFT_C2R;
+    if (!real && d.kind != AFFT_C2C) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "unknown transform kind");
+    double fscale = 1.0;
+    if (d.flags & AFFT_FLAG_NORMALIZE) fscale = 1.0 / (double)n;
+    if (d.flags & AFFT_FLAG_UNITARY) fscale = 1.0 / std::sqrt((double)n);
 
-    const int64_t is = d.istride ? d.istride : 1, os = d.ostride ? d.ostride : 1;
-    const int64_t id = d.idist ? d.idist : n * is, od = d.odist ? d.odist : n * os;
-    if (d.rank > 1 && (is != 1 || os != 1))
-        return set_error(AFFT_ERR_INVALID_STRIDE, "N-D plans take unit element stride");
-    // extent of the data each side addresses (elements)
-    p.in_elems[0] = p.in_elems[1] = (d.batch - 1) * id + (n - 1) * is + 1;
-    p.out_elems[0] = p.out_elems[1] = (d.batch - 1) * od + (n - 1) * os + 1;
-    p.in_esize[0] = p.in_esize[1] = p.out_esize[0] = p.out_esize[1] = es;
-
-    for (int di = 0; di < 2; ++di) {
-        std::string algo;
-        // axes last to first (plan_nd.go:405); the last axis reads src, the
-        // others work in place on dst
-        int64_t inner = 1;
-        for (int ax = d.rank - 1; ax >= 0; --ax) {
-            const int64_t len = d.dims[ax];
-            const bool first = ax == d.rank - 1;
-            if (len == 1) {
-                if (first && d.rank == 1) {
-                    // a 1-point transform is a copy (scaled by 1 either way)
-                    AxisJob j{};
-                    (void)j;
-                }
-                inner *= len;
-                continue;
-            }
-            AxisJob j{};
-            j.n = len;
-            j.scale = di ? 1.0 / (double)len : 1.0;
-            j.in_role = first ? ROLE_SRC : ROLE_DST;
-            j.out_role = ROLE_DST;
+    if (!real) {
+        const int64_t is = d.istride ? d.istride : 1, os = d.ostride ? d.ostride : 1;
+        const int64_t id = d.idist ? d.idist : n * is, od = d.odist ? d.odist : n * os;
+        if (d.rank > 1 && (is != 1 || os != 1))
+            return set_error(AFFT_ERR_INVALID_STRIDE, "N-D plans take unit element stride");
+        p.in_elems[0] = p.in_elems[1] = (d.batch - 1) * id + (n - 1) * is + 1;
+        p.out_elems[0] = p.out_elems[1] = (d.batch - 1) * od + (n - 1) * os + 1;
+        p.in_esize[0] = p.in_esize[1] = p.out_esize[0] = p.out_esize[1] = es;
+        for (int di = 0; di < 2; ++di) {
+            std::string algo;
             if (d.rank == 1) {
+                AxisJob j{};
+                j.n = n;
+                j.scale = di ? 1.0 / (double)n : fscale;
+                j.in_role = ROLE_SRC;
+                j.out_role = ROLE_DST;
                 j.count = d.batch;
                 j.slices = 1;
-                j.in_cd = id; j.in_nd = is; j.in_bd = 0;
-                j.out_cd = od; j.out_nd = os; j.out_bd = 0;
+                j.in_cd = id; j.in_nd = is;
+                j.out_cd = od; j.out_nd = os;
+                int rc = emit_axis(p, di, j, algo);
+                if (rc) return rc;
             } else {
-                // axis with stride `inner`; outer = everything before it, per batch
-                const int64_t outer = (n / (len * inner));
-                const bool src_side = first;
-                const int64_t bdist_in = src_side ? id : od;
-                (void)bdist_in;
-                if (inner == 1) {
-                    // rows: count = outer*batch when distances are packed
-                    if ((src_side ? id : od) == n && od == n) {
-                        j.count = outer * d.batch;
-                        j.slices = 1;
-                        j.in_cd = len; j.in_nd = 1; j.in_bd = 0;
-                        j.out_cd = len; j.out_nd = 1; j.out_bd = 0;
-                    } else {
-                        j.count = outer;
-                        j.slices = d.batch;
-                        j.in_cd = len; j.in_nd = 1; j.in_bd = src_side ? id : od;
-                        j.out_cd = len; j.out_nd = 1; j.out_bd = od;
-                    }
-                } else {
-                    j.count = inner;
-                    j.slices = outer * d.batch;
-                    if ((src_side ? id : od) != n || od != n)
-                        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "batched N-D with padded distance");
-                    j.in_cd = 1; j.in_nd = inner; j.in_bd = len * inner;
-                    j.out_cd = 1; j.out_nd = inner; j.out_bd = len * inner;
+                std::vector<int> axes;
+                for (int ax = d.rank - 1; ax >= 0; --ax) axes.push_back(ax);
+                if (n == 1) axes = {d.rank - 1};
+                int rc = emit_nd_axes(p, di, d.rank, d.dims, axes, d.batch, ROLE_SRC, id, ROLE_DST, od, algo);
+                if (rc) return rc;
+                if (p.steps[di].empty()) {
+                    // all axes of length 1 but src != dst: plain copy
+                    AxisJob j{};
+                    j.n = 1; j.count = d.batch * n; j.slices = 1;
+                    j.in_cd = 1; j.out_cd = 1; j.in_nd = j.out_nd = 1;
+                    j.in_role = ROLE_SRC; j.out_role = ROLE_DST; j.scale = 1.0;
+                    if ((rc = emit_axis(p, di, j, algo))) return rc;
                 }
+                if (!di && fscale != 1.0) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "normalised N-D forward");
             }
-            int rc = emit_axis(p, di, j, algo);
+            if (di == 0) p.algo = "hip_" + algo;
+        }
+        return AFFT_OK;
+    }
+
+    // ---- real transforms (PlanRealT, PlanReal2D/3D) --------------------------
+    const int64_t w = d.dims[d.rank - 1];
+    if (w < 2 || (w & 1)) return set_error(AFFT_ERR_INVALID_LENGTH, "real transforms need an even last dimension");
+    if ((d.istride && d.istride != 1) || (d.ostride && d.ostride != 1))
+        return set_error(AFFT_ERR_INVALID_STRIDE, "real transforms take unit element stride");
+    const int64_t h = w / 2, hw = h + 1;
+    const int64_t outer = n / w;
+    int64_t cd[8];
+    for (int i = 0; i < d.rank; ++i) cd[i] = d.dims[i];
+    cd[d.rank - 1] = hw;
+    const int64_t creal = n, ccx = outer * hw;
+    const int64_t rdist = d.idist ? d.idist : creal;   // real side, reals
+    const int64_t cdist = d.odist ? d.odist : ccx;     // complex side, complex elements
+    if (rdist & 1) return set_error(AFFT_ERR_INVALID_STRIDE, "real-side batch distance must be even");
+    // [0] forward: reals in, complex out; [1] inverse: complex in, reals out
+    p.in_elems[0] = (d.batch - 1) * rdist + creal;
+    p.out_elems[0] = (d.batch - 1) * cdist + ccx;
+    p.in_elems[1] = p.out_elems[0];
+    p.out_elems[1] = p.in_elems[0];
+    p.in_esize[0] = p.out_esize[1] = rs;
+    p.out_esize[0] = p.in_esize[1] = es;
+    const int64_t rd2 = rdist / 2;   // real batch distance in complex units
+    for (int di = 0; di < 2; ++di) {
+        std::string algo;
+        std::vector<int> axes;
+        for (int ax = d.rank - 2; ax >= 0; --ax) axes.push_back(ax);
+        int rc;
+        // rows are contiguous within one batch copy: outer rows of h / hw
+        const bool flat = (rdist == creal && cdist == ccx);
+        const int64_t rows = flat ? outer * d.batch : outer;
+        const int64_t slices = flat ? 1 : d.batch;
+        if (!di) {
+            rc = emit_real_rows(p, 0, w, rows, slices, ROLE_SRC, h, rd2, ROLE_DST, hw, cdist, fscale, algo);
             if (rc) return rc;
-            inner *= len;
+            if (!axes.empty()) {
+                if (fscale != 1.0) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "normalised N-D real forward");
+                rc = emit_nd_axes(p, 0, d.rank, cd, axes, d.batch, ROLE_DST, cdist, ROLE_DST, cdist, algo);
+                if (rc) return rc;
+            }
+            p.algo = "hip_" + algo;
+        } else {
+            int src_role = ROLE_SRC;
+            int64_t src_dist = cdist;
+            if (!axes.empty()) {
+                // outer axes first (inverse order of the forward), into WORK
+                std::vector<int> inv_axes(axes.rbegin(), axes.rend());
+                const size_t need = (size_t)(d.batch * ccx * es);
+                if (need > p.tmp_bytes[ROLE_WORK - 2]) p.tmp_bytes[ROLE_WORK - 2] = need;
+                rc = emit_nd_axes(p, 1, d.rank, cd, inv_axes, d.batch, ROLE_SRC, cdist, ROLE_WORK, ccx, algo);
+                if (rc) return rc;
+                src_role = ROLE_WORK;
+                src_dist = ccx;
+            }
+            const bool flat2 = (rdist == creal && src_dist == ccx);
+            const int64_t rows2 = flat2 ? outer * d.batch : outer;
+            const int64_t slices2 = flat2 ? 1 : d.batch;
+            rc = emit_real_rows(p, 1, w, rows2, slices2, src_role, hw, src_dist, ROLE_DST, h, rd2, 1.0, algo);
+            if (rc) return rc;
         }
-        if (p.steps[di].empty()) {
-            // every axis has length 1: a plain copy of each point
-            return set_error(AFFT_ERR_NOT_IMPLEMENTED, "all axes have length 1");
-        }
-        if (di == 0) p.algo = "hip_" + algo;
     }
     return AFFT_OK;
 }
 
 int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s) {
     const int di = dir == AFFT_INVERSE ? 1 : 0;
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < NTMP; ++t) {
         if (p.tmp_bytes[t] && !p.tmp[t]) AFFT_HIP(hipMalloc(&p.tmp[t], p.tmp_bytes[t]));
     }
-    void* base[ROLE_COUNT] = {const_cast<void*>(src), dst, p.tmp[0], p.tmp[1]};
+    void* base[ROLE_COUNT] = {const_cast<void*>(src), dst, p.tmp[0], p.tmp[1], p.tmp[2], p.tmp[3]};
     for (Step& st : p.steps[di]) {
         char* ip = (char*)base[st.in_role] + st.in_off;
         char* op = (char*)base[st.out_role] + st.out_off;

@@ -370,6 +370,27 @@ static int host_exec(afft_plan p, int dir, void* dst, int64_t dst_len, const voi
     if (p->steps[di].empty()) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "direction not available for this plan kind");
     if (src_len < p->in_elems[di] || dst_len < p->out_elems[di])
         return set_error(AFFT_ERR_LENGTH_MISMATCH, "slice shorter than the plan requires");
+    // PlanRealT.inverseSingle (plan_real_generic.go:298-316): DC and Nyquist
+    // bins must be real to 1e-4 (complex64) / 1e-12 (complex128)
+    if (di == 1 && p->desc.rank == 1 && (p->desc.kind == AFFT_R2C || p->desc.kind == AFFT_C2R)) {
+        const int64_t h = p->desc.dims[0] / 2;
+        const int64_t cd = p->desc.odist ? p->desc.odist : h + 1;
+        const double eps = p->desc.precision ? 1e-12 : 1e-4;
+        for (int64_t b = 0; b < p->desc.batch; ++b) {
+            double i0, ih;
+            if (p->desc.precision) {
+                const double* x = (const double*)src + 2 * b * cd;
+                i0 = x[1];
+                ih = x[2 * h + 1];
+            } else {
+                const float* x = (const float*)src + 2 * b * cd;
+                i0 = x[1];
+                ih = x[2 * h + 1];
+            }
+            if (std::fabs(i0) > eps || std::fabs(ih) > eps)
+                return set_error(AFFT_ERR_INVALID_SPECTRUM, "DC or Nyquist bin has a non-zero imaginary part");
+        }
+    }
     std::lock_guard<std::mutex> lock(p->mu);
     AFFT_HIP(hipSetDevice(p->ctx->device));
     size_t ib = (size_t)p->in_elems[di] * p->in_esize[di];
@@ -441,7 +462,9 @@ int afft_plan_describe(afft_plan p, char* buf, int cap) {
 
 int64_t afft_plan_workspace_bytes(afft_plan p) {
     if (!p) return 0;
-    return (int64_t)(p->tmp_bytes[0] + p->tmp_bytes[1]);
+    int64_t t = 0;
+    for (int i = 0; i < NTMP; ++i) t += (int64_t)p->tmp_bytes[i];
+    return t;
 }
 
 int afft_plan_destroy(afft_plan p) {

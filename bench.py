@@ -103,8 +103,11 @@ def main():
 
     ctx = A.Context(0)
     plan = A.Plan.from_desc(ctx, (n,), precision=A.COMPLEX64, batch=batch)
-    stream = torch.cuda.current_stream()
+    # a dedicated stream: the launches and the timing events share it
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     sh = stream.cuda_stream
+    assert sh, "need a non-null stream handle"
 
     def step():
         plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)

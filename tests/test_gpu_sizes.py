@@ -1,0 +1,112 @@
+"""Non-power-of-two sizes: the runtime mixed-radix path (2/3/5/7-smooth) and
+the Bluestein chain (all other sizes), against the oracle's mixed-radix and
+Bluestein restatements (internal/fft/mixedradix.go:45-358,
+plan_bluestein.go:7-58) on the same seeded inputs.
+
+Budgets against the FP64 oracle: complex64 rel L2 <= 2e-6*log2(N);
+complex128 rel L2 <= 1e-13.
+"""
+import numpy as np
+import pytest
+
+import algofft_hip as A
+from algofft_hip import synth
+import pyoracle as O
+from helpers import C128_TOL, c64_tol, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SMOOTH = [3, 5, 6, 7, 9, 12, 15, 30, 49, 100, 243, 360, 1000, 2187, 2401, 3000, 3125, 6000, 8000, 12000, 59049]
+PRIMEISH = [11, 13, 17, 31, 101, 251, 509, 1009, 4093, 8191, 8209, 65521, 10007 * 3]
+
+
+def _plan_run(ctx, x, inverse=False, flags=0, **kw):
+    prec = A.COMPLEX64 if x.dtype == np.complex64 else A.COMPLEX128
+    batch, n = x.shape
+    with A.Plan.from_desc(ctx, (n,), precision=prec, batch=batch, flags=flags, **kw) as p:
+        out = np.empty_like(x)
+        (p.inverse if inverse else p.forward)(out, x)
+        return out, p.describe()
+
+
+@pytest.mark.parametrize("n", SMOOTH + PRIMEISH)
+def test_c64_forward(ctx, n):
+    batch = max(1, min(16, 200000 // n))
+    x = synth.uniform_complex(n, (batch, n), np.complex64)
+    got, algo = _plan_run(ctx, x)
+    ref = O.fft_batch(x.astype(np.complex128))
+    err = rel_l2(got, ref)
+    print(f"n={n} algo={algo} relL2={err:.2e}")
+    assert err <= c64_tol(n), (n, err, algo)
+
+
+@pytest.mark.parametrize("n", [3, 7, 100, 2187, 3125, 12000, 13, 509, 8191, 65521])
+def test_c128_forward_inverse(ctx, n):
+    batch = max(1, min(8, 100000 // n))
+    x = synth.uniform_complex(7 * n, (batch, n), np.complex128)
+    got, algo = _plan_run(ctx, x)
+    ref = O.fft_batch(x)
+    assert rel_l2(got, ref) <= C128_TOL, (n, rel_l2(got, ref), algo)
+    back, _ = _plan_run(ctx, got, inverse=True)
+    ref_b = O.fft_batch(got, inverse=True)
+    assert rel_l2(back, ref_b) <= C128_TOL
+    assert rel_l2(back, x) <= 2 * C128_TOL
+
+
+@pytest.mark.parametrize("n", [6, 360, 3000, 12000, 17, 1009, 8209])
+def test_c64_inverse(ctx, n):
+    x = synth.uniform_complex(3 * n, (4, n), np.complex64)
+    got, algo = _plan_run(ctx, x, inverse=True)
+    ref = O.fft_batch(x.astype(np.complex128), inverse=True)
+    assert rel_l2(got, ref) <= c64_tol(n), (n, rel_l2(got, ref), algo)
+
+
+def test_length_one(ctx):
+    x = synth.uniform_complex(1, (5, 1), np.complex64)
+    got, algo = _plan_run(ctx, x)
+    assert np.array_equal(got, x)
+    back, _ = _plan_run(ctx, x, inverse=True)
+    assert np.array_equal(back, x)
+
+
+@pytest.mark.parametrize("n", [1009, 8209])
+def test_reference_chirp_flag(ctx, n):
+    # the unreduced chirp of bluestein.go:23-28 is selectable; at these sizes
+    # both chirps agree to rounding, and both match the oracle's matching form
+    x = synth.uniform_complex(5, (2, n), np.complex128)
+    got, _ = _plan_run(ctx, x, flags=A.FLAG_REF_CHIRP)
+    ref = O.fft_batch(x, exact_chirp=False)
+    assert rel_l2(got, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("n,stride", [(360, 7), (17, 40), (8209, 3), (12000, 2)])
+def test_strided_mixed_and_bluestein(ctx, n, stride):
+    buf = synth.uniform_complex(n + stride, ((n - 1) * stride + 1,), np.complex64)
+    with A.Plan.from_desc(ctx, (n,), precision=A.COMPLEX64, istride=stride, ostride=stride) as p:
+        out = np.zeros_like(buf)
+        p.forward(out, buf)
+    ref = O.fft_strided(buf.astype(np.complex128), n, stride)
+    idx = np.arange(n) * stride
+    assert rel_l2(out[idx], ref[idx]) <= c64_tol(n)
+
+
+@pytest.mark.parametrize("dims", [(6, 10), (7, 9, 11), (3, 5, 64), (100, 1), (1, 17)])
+def test_nd_mixed(ctx, dims):
+    x = synth.uniform_complex(11, dims, np.complex128)
+    with A.Plan.from_desc(ctx, dims, precision=A.COMPLEX128) as p:
+        out = np.empty_like(x)
+        p.forward(out, x)
+        back = np.empty_like(x)
+        p.inverse(back, out)
+    assert rel_l2(out, O.fftn(x)) <= C128_TOL
+    assert rel_l2(back, x) <= 2 * C128_TOL
+
+
+def test_interleaved_columns_mixed(ctx):
+    n, cols = 360, 33
+    a = synth.uniform_complex(9, (n, cols), np.complex64)
+    with A.Plan.from_desc(ctx, (n,), batch=cols, istride=cols, idist=1, ostride=cols, odist=1) as p:
+        out = np.empty_like(a)
+        p.forward(out, a)
+    ref = O.fft_batch(np.ascontiguousarray(a.T).astype(np.complex128)).T
+    assert rel_l2(out, ref) <= c64_tol(n)
