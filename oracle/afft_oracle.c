@@ -82,6 +82,16 @@ int or_dft_naive(long n, const double *in, double *out, int inverse) {
     return 0;
 }
 
+/* Planted-mistake switch for the tests that show the pinning checks catch
+ * plausible errors (0 = none):
+ *   1  forward twiddle sign flipped in the Stockham stages
+ *   2  last term of every mixed-radix butterfly dropped
+ *   3  Bluestein chirp index off by one
+ *   4  N-D axes swapped (transposed operand)
+ * Only the tests set it. */
+int or_fault = 0;
+void or_set_fault(int f) { or_fault = f; }
+
 /* internal/math/factor.go:31-43 */
 int or_is_highly_composite(long n) {
     if (n <= 0) return 0;

@@ -171,3 +171,8 @@ def rng_u01(seed, count, offset=0):
     out = np.empty(count, dtype=np.float64)
     lib().or_rng_u01(seed, offset, count, _p(out))
     return out
+
+
+def set_fault(f):
+    """Plant a known mistake in the oracle (0 = none); see afft_oracle.c."""
+    lib().or_set_fault(int(f))
