@@ -259,3 +259,48 @@ class Plan:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Ragged:
+    """A ragged batch: complex transforms of lengths `sizes` packed back to
+    back, run by one persistent launch (exec), its captured graph
+    (exec_graph) or one plan execution per job (exec_per_size)."""
+
+    def __init__(self, ctx, sizes, precision=COMPLEX64, flags=0):
+        arr = (ctypes.c_int64 * len(sizes))(*[int(v) for v in sizes])
+        h = ctypes.c_void_p()
+        check(lib().afft_ragged_create(ctx.h, precision, arr, len(sizes), flags, ctypes.byref(h)))
+        self.h = h
+        self.sizes = [int(v) for v in sizes]
+
+    def __len__(self):
+        return lib().afft_ragged_total_len(self.h)
+
+    def offsets(self):
+        out = (ctypes.c_int64 * len(self.sizes))()
+        check(lib().afft_ragged_offsets(self.h, out, len(self.sizes)))
+        return list(out)
+
+    def _x(self, fn, direction, dst_ptr, src_ptr, stream):
+        check(fn(self.h, direction, ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr),
+                 ctypes.c_void_p(stream) if stream else None))
+
+    def exec(self, direction, dst_ptr, src_ptr, stream=None):
+        self._x(lib().afft_ragged_exec, direction, dst_ptr, src_ptr, stream)
+
+    def exec_graph(self, direction, dst_ptr, src_ptr, stream=None):
+        self._x(lib().afft_ragged_exec_graph, direction, dst_ptr, src_ptr, stream)
+
+    def exec_per_size(self, direction, dst_ptr, src_ptr, stream=None):
+        self._x(lib().afft_ragged_exec_per_size, direction, dst_ptr, src_ptr, stream)
+
+    def close(self):
+        if self.h:
+            check(lib().afft_ragged_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -90,9 +90,26 @@ struct Plan {
 };
 
 int build_plan(Plan& p);
+
+// Helpers shared by the planners (plan.cpp).
+int upload_table(Plan& p, const void* host, size_t bytes, const void** dev);
+std::vector<int> smooth_schedule(int64_t n);   // empty unless 2/3/5/7-smooth
+int64_t next_pow2(int64_t n);
+struct BlueTables {
+    const void* chirp = nullptr;                 // n entries, forward sign
+    const void* filt[2] = {nullptr, nullptr};    // [fwd, inv], m entries, 1/m folded in
+    const void* post[2] = {nullptr, nullptr};    // [fwd, inv]: chirp, conj(chirp)*inv_scale
+};
+int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt);
 int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s);
 
 // FP64 roots of unity W_L^m = exp(-2πi m / L), exact integer reduction of m.
 void root_of_unity(int64_t m, int64_t L, double* re, double* im);
 
 }  // namespace afft
+
+// The opaque handle types of the C ABI are the internal objects.
+struct afft_context_s : afft::Context {};
+struct afft_buffer_s : afft::Buffer {};
+struct afft_stream_s : afft::Stream {};
+struct afft_plan_s : afft::Plan {};

@@ -88,8 +88,29 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
             const int j = t + TT * b;
             const int k = j % NS;
             const cx<T>* __restrict__ w = tw + S::twoff(p) + k * (R - 1);
+            if constexpr (R == 16 || R == 8) {
+                // Load w^1, w^2, w^4 (, w^8) and form the other powers as at
+                // most three products: a quarter of the table traffic, at a
+                // cost of a few ulps on the composite powers.
+                cx<T> ww[R];
+                ww[1] = w[0];
+                ww[2] = w[1];
+                ww[4] = w[3];
+                ww[3] = cmul(ww[1], ww[2]);
+                ww[5] = cmul(ww[1], ww[4]);
+                ww[6] = cmul(ww[2], ww[4]);
+                ww[7] = cmul(ww[3], ww[4]);
+                if constexpr (R == 16) {
+                    ww[8] = w[7];
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], w[r - 1]);
+                    for (int r = 9; r < 16; ++r) ww[r] = cmul(ww[r - 8], ww[8]);
+                }
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], ww[r]);
+            } else {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], w[r - 1]);
+            }
         }
     }
 #pragma unroll

@@ -37,7 +37,7 @@ static int ilog2(int64_t n) {
 }
 
 // Upload a host table, owned by the plan.
-static int upload_table(Plan& p, const void* host, size_t bytes, const void** dev) {
+int upload_table(Plan& p, const void* host, size_t bytes, const void** dev) {
     void* d = nullptr;
     AFFT_HIP(hipMalloc(&d, bytes ? bytes : 16));
     p.owned.push_back(d);
@@ -251,7 +251,7 @@ const void* copy_kernel(int prec);
 
 // Radix schedule for a 2/3/5/7-smooth n: radix-16 passes, the power-of-two
 // remainder, then 3s, 5s and 7s.  Empty if n has another prime factor.
-static std::vector<int> smooth_schedule(int64_t n) {
+std::vector<int> smooth_schedule(int64_t n) {
     std::vector<int> r;
     int64_t m = n;
     int a = 0;
@@ -268,7 +268,7 @@ static std::vector<int> smooth_schedule(int64_t n) {
 
 static int gen_cap(int prec) { return prec ? 4096 : 8192; }
 
-static int64_t next_pow2(int64_t n) {
+int64_t next_pow2(int64_t n) {
     int64_t m = 1;
     while (m < n) m <<= 1;
     return m;
@@ -476,16 +476,10 @@ static void host_fft_ld(std::vector<long double>& re, std::vector<long double>& 
     }
 }
 
-struct BlueTables {
-    const void* chirp = nullptr;     // N, forward sign
-    const void* filt[2] = {nullptr, nullptr};  // [fwd, inv], 1/M folded in
-    const void* post[2] = {nullptr, nullptr};  // [fwd, inv]: chirp, conj(chirp)*scale
-};
-
 // chirp[k] = exp(-iπ k²/N); with exact reduction k² mod 2N in integers
 // unless the reference's unreduced formula is requested
 // (internal/kernels/bluestein.go:20-32).
-static int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt) {
+int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt) {
     const bool refchirp = p.desc.flags & AFFT_FLAG_REF_CHIRP;
     std::vector<double> ch(2 * n);
     for (int64_t k = 0; k < n; ++k) {

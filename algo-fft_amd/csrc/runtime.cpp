@@ -80,10 +80,6 @@ void root_of_unity(int64_t m, int64_t L, double* re, double* im) {
 using namespace afft;
 
 // ---- C ABI ---------------------------------------------------------------------
-struct afft_context_s : Context {};
-struct afft_buffer_s : Buffer {};
-struct afft_stream_s : Stream {};
-struct afft_plan_s : Plan {};
 
 static void copy_str(char* dst, size_t cap, const char* src) {
     if (!cap) return;

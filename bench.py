@@ -51,6 +51,17 @@ def gen_input_device(n, batch, seed, torch):
     return out
 
 
+def traffic_bytes(n, batch):
+    """HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    for this workload, scaled by the same counters on a device copy of known
+    size (profiles/traffic_*.json, written from those runs), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_c64_n{n}_b{batch}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
 def cpu_baseline(n, seconds=10.0):
     """CPU restatement of the reference's complex64 ForwardBatch (oracle),
     sharded over host threads; timed on a bounded sample."""
@@ -115,6 +126,7 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # whole-job timing: one event pair around the K steps
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
@@ -124,6 +136,28 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = ev0.elapsed_time(ev1) / args.steps
+    # per-launch durations (same stream): the roofline uses their average
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    for i in range(args.steps):
+        evs[2 * i].record(stream)
+        step()
+        evs[2 * i + 1].record(stream)
+    torch.cuda.synchronize()
+    launch_ms = [evs[2 * i].elapsed_time(evs[2 * i + 1]) for i in range(args.steps)]
+    kms = float(np.mean(launch_ms))
+    # device copy of the same bytes: the practical streaming ceiling
+    cps = []
+    for i in range(5):
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        y.copy_(x)
+        c1.record(stream)
+        torch.cuda.synchronize()
+        if i:
+            cps.append(c0.elapsed_time(c1))
+    copy_gbs = 2 * x.numel() * x.element_size() / (np.mean(cps) * 1e-3) / 1e9
+    step()
+    torch.cuda.synchronize()
 
     # accuracy on sampled rows against numpy's FP64 FFT of the same input
     rows = np.linspace(0, batch - 1, 16).astype(np.int64)
@@ -135,7 +169,7 @@ def main():
     flops = 5.0 * n * np.log2(n) * batch
     gflops = flops / (ms * 1e-3) / 1e9
     alg_bytes = 2 * 8 * n * batch
-    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    gbs = alg_bytes / (kms * 1e-3) / 1e9
     nsteps = len(plan.describe().split("+"))
     out = {
         "metric": "batched C2C GFLOP/s + % HBM peak, MFMA/ragged vs plain GPU at rel L2<=2e-6*log2N",
@@ -156,8 +190,10 @@ def main():
         "rel_l2_budget": 2e-6 * np.log2(n),
         "wall_s_timed": round(wall, 4),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_launches_per_step": nsteps},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(n, batch),
+                     "kernel_ms": round(kms, 5), "kernel_launches_per_step": nsteps,
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "device_copy_gbs": round(copy_gbs, 1), "frac_of_device_copy": round(gbs / copy_gbs, 4)},
     }
     plan.close()
     ctx.close()

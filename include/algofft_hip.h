@@ -167,6 +167,26 @@ int afft_plan_describe(afft_plan p, char* buf, int cap);
 int64_t afft_plan_workspace_bytes(afft_plan p);
 int afft_plan_destroy(afft_plan p);           /* PlanImpl.Close */
 
+/* ---- ragged batches ------------------------------------------------------
+ * No single counterpart in the reference: a caller there loops Plan.Forward
+ * (plan.go:332) over plans of different sizes, as ForwardBatch loops one size
+ * (plan.go:504-530).  Here `count` complex transforms of lengths sizes[i]
+ * are packed back to back (job i starts at the sum of the earlier lengths)
+ * and run as one persistent work-queue launch. */
+typedef struct afft_ragged_s* afft_ragged;
+int afft_ragged_create(afft_context ctx, int precision, const int64_t* sizes, int64_t count, int flags,
+                       afft_ragged* out);
+int64_t afft_ragged_total_len(afft_ragged r);                    /* elements of the packed layout */
+int afft_ragged_offsets(afft_ragged r, int64_t* offsets, int64_t cap);
+/* One launch of the persistent kernel (out of place, device pointers). */
+int afft_ragged_exec(afft_ragged r, int direction, void* dst, const void* src, void* stream);
+/* The same launch captured once per (direction, dst, src, stream) in a
+ * hipGraph and replayed. */
+int afft_ragged_exec_graph(afft_ragged r, int direction, void* dst, const void* src, void* stream);
+/* Every job through the plan of its own size: one plan execution per job. */
+int afft_ragged_exec_per_size(afft_ragged r, int direction, void* dst, const void* src, void* stream);
+int afft_ragged_destroy(afft_ragged r);
+
 #ifdef __cplusplus
 }
 #endif
