@@ -60,6 +60,7 @@ SIGNATURES = [
     ("afft_plan_inverse", c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_i64]),
     ("afft_plan_exec_device", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("afft_plan_exec_buffers", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("afft_plan_synchronize", c_int, [c_void_p]),
     ("afft_plan_describe", c_int, [c_void_p, ctypes.c_char_p, c_int]),
     ("afft_plan_workspace_bytes", c_i64, [c_void_p]),
     ("afft_plan_destroy", c_int, [c_void_p]),

@@ -70,3 +70,59 @@ def sinusoids(seed, n, batch, dtype=np.complex64, tones=8, noise=1e-3):
         sig += amps[:, j:j + 1] * np.exp(1j * ph)
     sig += normal_complex(seed + 1, (batch, n)) * noise
     return sig.astype(dtype), bins, amps
+
+
+def _is_smooth7(n):
+    for q in (2, 3, 5, 7):
+        while n % q == 0:
+            n //= q
+    return n == 1
+
+
+def _is_prime(n):
+    if n < 2:
+        return False
+    if n % 2 == 0:
+        return n == 2
+    f = 3
+    while f * f <= n:
+        if n % f == 0:
+            return False
+        f += 2
+    return True
+
+
+def ragged_sizes(seed, count=4096, lo=16, hi=65536):
+    """Ragged-batch lengths: log-uniform in [lo, hi]; 60% rounded to a power
+    of two, 30% to the nearest 2^a 3^b 5^c 7^d, 10% to the nearest prime
+    (the Bluestein sizes)."""
+    u = u01(seed, 2 * count).reshape(count, 2)
+    out = []
+    for i in range(count):
+        v = int(round(np.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * u[i, 0])))
+        v = min(max(v, lo), hi)
+        c = u[i, 1]
+        if c < 0.6:
+            n = 1 << int(round(np.log2(v)))
+        elif c < 0.9:
+            d = 0
+            while True:
+                if v - d >= lo and _is_smooth7(v - d):
+                    n = v - d
+                    break
+                if v + d <= hi and _is_smooth7(v + d):
+                    n = v + d
+                    break
+                d += 1
+        else:
+            d = 0
+            while True:
+                if _is_prime(v - d) and v - d >= 2:
+                    n = v - d
+                    break
+                if _is_prime(v + d):
+                    n = v + d
+                    break
+                d += 1
+        out.append(min(max(n, 2), 1 << 20))
+    return out

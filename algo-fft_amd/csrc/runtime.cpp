@@ -449,6 +449,14 @@ int afft_plan_exec_buffers(afft_plan p, int direction, afft_buffer dst, afft_buf
     return afft_plan_exec_device(p, direction, dst->ptr, src->ptr, s ? (void*)s->s : nullptr);
 }
 
+int afft_plan_synchronize(afft_plan p) {
+    if (!p) return set_error(AFFT_ERR_INVALID_ARGUMENT, "NULL plan");
+    AFFT_HIP(hipSetDevice(p->ctx->device));
+    AFFT_HIP(hipStreamSynchronize(p->stream));
+    clear_error();
+    return AFFT_OK;
+}
+
 int afft_plan_describe(afft_plan p, char* buf, int cap) {
     if (!p || !buf || cap <= 0) return set_error(AFFT_ERR_INVALID_ARGUMENT, "NULL argument");
     copy_str(buf, (size_t)cap, p->algo.c_str());

@@ -62,6 +62,47 @@ def traffic_bytes(n, batch):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
+def ragged_leg(torch, A, ctx, stream, steps):
+    """Mixed-size workload: 4096 complex64 transforms, lengths log-uniform in
+    [16, 65536] (60% powers of two, 30% 7-smooth, 10% primes via Bluestein),
+    packed back to back.  Persistent work-queue launch and its graph replay
+    against one plan execution per job."""
+    from algofft_hip import synth
+    sizes = synth.ragged_sizes(1234, 4096)
+    rg = A.Ragged(ctx, sizes, A.COMPLEX64)
+    total = len(rg)
+    g = torch.Generator(device="cuda:0").manual_seed(5)
+    x = torch.complex(torch.rand(total, device="cuda:0", generator=g) * 2 - 1,
+                      torch.rand(total, device="cuda:0", generator=g) * 2 - 1)
+    y = torch.empty_like(x)
+    sh = stream.cuda_stream
+
+    def timed(fn, k):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    ms_p = timed(lambda: rg.exec(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), steps)
+    ms_g = timed(lambda: rg.exec_graph(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), steps)
+    ms_s = timed(lambda: rg.exec_per_size(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), max(2, steps // 10))
+    flops = float(sum(5.0 * n * np.log2(n) for n in sizes))
+    alg = 16.0 * total
+    rg.close()
+    return {"workload": "4096 ragged complex64 forward transforms, N log-uniform 16..65536 "
+                        "(60% pow2 / 30% 7-smooth / 10% prime)",
+            "points": total, "distinct_sizes": len(set(sizes)),
+            "persistent_ms": round(ms_p, 4), "graph_ms": round(ms_g, 4), "per_size_launch_ms": round(ms_s, 4),
+            "gflops_graph": round(flops / (ms_g * 1e-3) / 1e9, 1),
+            "gbs_graph_algorithmic": round(alg / (ms_g * 1e-3) / 1e9, 1),
+            "speedup_graph_vs_per_size": round(ms_s / ms_g, 2)}
+
+
 def cpu_baseline(n, seconds=10.0):
     """CPU restatement of the reference's complex64 ForwardBatch (oracle),
     sharded over host threads; timed on a bounded sample."""
@@ -96,6 +137,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="default 2^26/N")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-ragged", action="store_true", help="skip the mixed-size leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -196,6 +238,8 @@ def main():
                      "device_copy_gbs": round(copy_gbs, 1), "frac_of_device_copy": round(gbs / copy_gbs, 4)},
     }
     plan.close()
+    if not args.no_ragged:
+        out["ragged"] = ragged_leg(torch, A, ctx, stream, max(3, args.steps // 5))
     ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)

@@ -161,6 +161,9 @@ int afft_plan_inverse(afft_plan p, void* dst, int64_t dst_len, const void* src, 
  * and returns without synchronising. */
 int afft_plan_exec_device(afft_plan p, int direction, void* dst_dev, const void* src_dev, void* stream);
 int afft_plan_exec_buffers(afft_plan p, int direction, afft_buffer dst, afft_buffer src, afft_stream s);
+/* Waits for the work enqueued on the plan's own stream (exec calls made with
+ * a NULL stream). */
+int afft_plan_synchronize(afft_plan p);
 /* Algorithm string recorded in wisdom (cf. Plan.Algorithm, plan.go:157). */
 int afft_plan_describe(afft_plan p, char* buf, int cap);
 /* Bytes of device workspace the plan holds. */
