@@ -27,6 +27,7 @@ COMPLEX64, COMPLEX128 = 0, 1
 FORWARD, INVERSE = -1, 1
 C2C, R2C, C2R = 0, 1, 2
 FLAG_INPLACE, FLAG_MFMA, FLAG_NORMALIZE, FLAG_UNITARY, FLAG_REF_CHIRP = 1, 2, 4, 8, 16
+STRATEGY_AUTO, STRATEGY_STOCKHAM, STRATEGY_SIXSTEP, STRATEGY_BLUESTEIN, STRATEGY_MIXED = 0, 2, 3, 5, 16
 
 
 class AfftError(Exception):
@@ -190,7 +191,7 @@ class Plan:
 
     @classmethod
     def from_desc(cls, ctx, dims, precision=COMPLEX64, kind=C2C, batch=1, istride=0, idist=0, ostride=0,
-                  odist=0, flags=0):
+                  odist=0, flags=0, strategy=0, split=0):
         if isinstance(dims, int):
             dims = (dims,)
         d = PlanDesc()
@@ -200,6 +201,7 @@ class Plan:
         for i, v in enumerate(dims):
             d.dims[i] = v
         d.batch, d.istride, d.idist, d.ostride, d.odist, d.flags = batch, istride, idist, ostride, odist, flags
+        d.strategy, d.split = strategy, split
         h = ctypes.c_void_p()
         check(lib().afft_plan_create_desc(ctx.h, ctypes.byref(d), ctypes.byref(h)))
         return cls(h, d)
@@ -304,3 +306,16 @@ class Ragged:
 
     def __exit__(self, *a):
         self.close()
+
+
+def convolve(ctx, a, b, correlate=False):
+    """Convolve / CrossCorrelate (convolve.go:9-59, correlate.go:13-33) of two
+    host arrays on the device; returns len(a)+len(b)-1 values."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    prec = _np_prec(a.dtype)
+    out = np.empty(a.size + b.size - 1, dtype=a.dtype)
+    check(lib().afft_convolve_host(ctx.h, prec, int(correlate), out.ctypes.data_as(ctypes.c_void_p), out.size,
+                                   a.ctypes.data_as(ctypes.c_void_p), a.size, b.ctypes.data_as(ctypes.c_void_p),
+                                   b.size))
+    return out

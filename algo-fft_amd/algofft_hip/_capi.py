@@ -26,7 +26,8 @@ class PlanOptions(ctypes.Structure):
 
 class PlanDesc(ctypes.Structure):
     _fields_ = [("kind", c_int), ("precision", c_int), ("rank", c_int), ("dims", c_i64 * 8), ("batch", c_i64),
-                ("istride", c_i64), ("idist", c_i64), ("ostride", c_i64), ("odist", c_i64), ("flags", c_int)]
+                ("istride", c_i64), ("idist", c_i64), ("ostride", c_i64), ("odist", c_i64), ("flags", c_int),
+                ("strategy", c_int), ("split", c_i64)]
 
 
 # (name, restype, argtypes) for every exported entry point of the header.
@@ -71,6 +72,34 @@ SIGNATURES = [
     ("afft_ragged_exec_graph", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("afft_ragged_exec_per_size", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("afft_ragged_destroy", c_int, [c_void_p]),
+    ("afft_plan_candidates", c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(c_int), ctypes.POINTER(c_i64), c_int]),
+    ("afft_plan_features", c_int, [ctypes.POINTER(PlanDesc), c_int, c_i64, ctypes.POINTER(ctypes.c_float)]),
+    ("afft_tune_time", c_int, [c_void_p, ctypes.POINTER(PlanDesc), c_int, c_int, ctypes.POINTER(ctypes.c_double)]),
+    ("afft_tune_plan", c_int, [c_void_p, ctypes.POINTER(PlanDesc), c_int, c_int, ctypes.POINTER(c_int),
+                               ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               c_int]),
+    ("afft_wisdom_export", c_int, [c_void_p, ctypes.c_char_p, c_i64, ctypes.POINTER(c_i64)]),
+    ("afft_wisdom_import", c_int, [c_void_p, ctypes.c_char_p]),
+    ("afft_wisdom_clear", c_int, [c_void_p]),
+    ("afft_mlp_param_count", c_int, []),
+    ("afft_mlp_create", c_int, [c_void_p, ctypes.c_uint64, ctypes.POINTER(c_void_p)]),
+    ("afft_mlp_set_normalization", c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                           ctypes.c_float, ctypes.c_float]),
+    ("afft_mlp_get_params", c_int, [c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("afft_mlp_set_params", c_int, [c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("afft_mlp_loss_grad", c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), c_i64,
+                                   ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    ("afft_mlp_train", c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), c_i64, c_int,
+                               c_int, ctypes.c_float, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]),
+    ("afft_mlp_predict", c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), c_i64, ctypes.POINTER(ctypes.c_float)]),
+    ("afft_mlp_save", c_int, [c_void_p, ctypes.c_char_p]),
+    ("afft_mlp_load", c_int, [c_void_p, ctypes.c_char_p]),
+    ("afft_mlp_destroy", c_int, [c_void_p]),
+    ("afft_context_set_cost_model", c_int, [c_void_p, c_void_p]),
+    ("afft_conv_create", c_int, [c_void_p, c_int, c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_void_p)]),
+    ("afft_conv_exec", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("afft_conv_destroy", c_int, [c_void_p]),
+    ("afft_convolve_host", c_int, [c_void_p, c_int, c_int, c_void_p, c_i64, c_void_p, c_i64, c_void_p, c_i64]),
 ]
 
 _lib = None

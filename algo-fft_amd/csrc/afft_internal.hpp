@@ -4,9 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/algofft_hip.h"
@@ -31,9 +33,25 @@ const KernelInfo* find_kernel(int prec, int n, int var, int dir);
 int max_single_pass(int prec);        // largest power of two held in LDS
 
 // ---- objects ----------------------------------------------------------------
+// One wisdom record: the reference's WisdomEntry (planner/wisdom.go:17-30),
+// keyed by size, precision and a device feature word.
+struct WisdomEntry {
+    int64_t size = 0;
+    int precision = 0;
+    uint64_t features = 0;
+    std::string algorithm;
+    int64_t timestamp = 0;
+};
+
+struct Mlp;
+
 struct Context {
     int device = 0;
     hipDeviceProp_t props{};
+    std::mutex mu;                                                   // guards wisdom and model
+    std::map<std::tuple<int64_t, int, uint64_t>, WisdomEntry> wisdom;
+    Mlp* model = nullptr;                                            // cost model, not owned
+    uint64_t features() const;                                       // gfx arch number, e.g. 0x950
 };
 
 struct Buffer {
@@ -101,6 +119,23 @@ struct BlueTables {
     const void* post[2] = {nullptr, nullptr};    // [fwd, inv]: chirp, conj(chirp)*inv_scale
 };
 int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt);
+int max_single_pass(int prec);
+int gen_cap(int prec);
+
+// ---- planning knowledge (tune.cpp, mlp.hip) --------------------------------------
+constexpr int NFEAT = 32;
+struct Candidate {
+    int strategy;
+    int64_t split;
+};
+// Strategies worth timing for a 1-D transform of length n.
+std::vector<Candidate> plan_candidates(const afft_plan_desc& d);
+// Feature vector of (shape, candidate) for the cost model.
+void plan_features(const afft_plan_desc& d, const Candidate& c, float* out);
+std::string strategy_name(const Candidate& c);
+bool parse_strategy(const std::string& algo, Candidate* c);
+// Predicted log-runtimes of candidates (GPU inference); 0 on success.
+int mlp_predict_host(Mlp* m, const float* x, int64_t n, float* y);
 int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s);
 
 // FP64 roots of unity W_L^m = exp(-2πi m / L), exact integer reduction of m.

@@ -39,6 +39,46 @@ __device__ __forceinline__ void dft_rt(int R, cx<T>* v) {
     }
 }
 
+// One runtime Stockham pass with a compile-time radix R, so the butterfly
+// stays in registers: thread t of T handles butterflies j = t, t+T, ... of
+// the N/R in the pass; load(n) gives input point n, store(k, v) takes output
+// point k = (j/Ns)*Ns*R + (j mod Ns) + r*Ns.
+template <int R, int DIR, typename T, class Load, class Store>
+__device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, const cx<T>* __restrict__ tw, Load load,
+                                        Store store) {
+    const int NR = N / R;
+    for (int j = t; j < NR; j += TT) {
+        cx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = load(j + r * NR);
+        const int k = j % NS;
+        if (NS > 1) {
+            const cx<T>* w = tw + k * (R - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], w[r - 1]);
+        }
+        dft<R, DIR, 1>(v);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) store(base + r * NS, v[r]);
+    }
+}
+
+template <int DIR, typename T, class Load, class Store>
+__device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS, const cx<T>* tw, Load load,
+                                            Store store) {
+    switch (R) {
+    case 2: rt_pass<2, DIR>(t, TT, N, NS, tw, load, store); break;
+    case 3: rt_pass<3, DIR>(t, TT, N, NS, tw, load, store); break;
+    case 4: rt_pass<4, DIR>(t, TT, N, NS, tw, load, store); break;
+    case 5: rt_pass<5, DIR>(t, TT, N, NS, tw, load, store); break;
+    case 7: rt_pass<7, DIR>(t, TT, N, NS, tw, load, store); break;
+    case 8: rt_pass<8, DIR>(t, TT, N, NS, tw, load, store); break;
+    case 16: rt_pass<16, DIR>(t, TT, N, NS, tw, load, store); break;
+    default: break;
+    }
+}
+
 // VAR_ROW / VAR_COL / VAR_4A / VAR_4B as in kernels.hpp.  ROW-like variants
 // lay a transform out contiguously in LDS, COL variants interleave G columns.
 template <typename T, int VAR, int DIR>
@@ -69,43 +109,32 @@ __global__ void __launch_bounds__(GEN_THREADS) k_gen(GArgs g) {
     for (int p = 0; p < g.npass; ++p) {
         const int R = g.rad[p];
         const int NS = g.ns[p];
-        const int NR = N / R;
         const bool last = p == g.npass - 1;
-        for (int j = t; j < NR; j += TPT) {
-            cx<T> v[16];
-            for (int r = 0; r < R; ++r) {
-                const int n = j + r * NR;
-                if (p == 0) v[r] = act ? src_g[(long long)n * ind] : mk<T>(0, 0);
-                else v[r] = sm[lidx((p - 1) & 1, n)];
-            }
-            if (NS > 1) {
-                const cx<T>* w = tw + g.twoff[p] + (j % NS) * (R - 1);
-                for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], w[r - 1]);
-            }
-            dft_rt<DIR>(R, v);
+        const cx<T>* twp = tw + g.twoff[p];
+        const int rb = (p - 1) & 1, wb = p & 1;
+        auto load = [&](int n) -> cx<T> {
+            if (p == 0) return act ? src_g[(long long)n * ind] : mk<T>(0, 0);
+            return sm[lidx(rb, n)];
+        };
+        auto store = [&](int k, cx<T> y) {
             if (!last) {
-                const int base = (j / NS) * NS * R + (j % NS);
-                for (int r = 0; r < R; ++r) sm[lidx(p & 1, base + r * NS)] = v[r];
-            } else {
-                // output index k = j + r*NS (natural order)
-                for (int r = 0; r < R; ++r) {
-                    const int k = j + r * NS;
-                    cx<T> y = v[r];
-                    if constexpr (VAR == VAR_4A) {
-                        const long long m = (long long)k * (cg + a.tw4_c0);
-                        y = twmul<DIR>(y, tw4<T>(a, m));
-                    }
-                    if (a.scale != 1.0) y = scl(y, (T)a.scale);
-                    if constexpr (VAR == VAR_ROW) {
-                        if (act) out[cg * a.out_cd + k] = y;
-                    } else if constexpr (VAR == VAR_COL || VAR == VAR_4A) {
-                        if (act) out[cg + (long long)k * a.out_nd] = y;
-                    } else {
-                        sm[lidx(p & 1, k)] = y;   // re-mapped below
-                    }
-                }
+                sm[lidx(wb, k)] = y;
+                return;
             }
-        }
+            if constexpr (VAR == VAR_4A) {
+                const long long m = (long long)k * (cg + a.tw4_c0);
+                y = twmul<DIR>(y, tw4<T>(a, m));
+            }
+            if (a.scale != 1.0) y = scl(y, (T)a.scale);
+            if constexpr (VAR == VAR_ROW) {
+                if (act) out[cg * a.out_cd + k] = y;
+            } else if constexpr (VAR == VAR_COL || VAR == VAR_4A) {
+                if (act) out[cg + (long long)k * a.out_nd] = y;
+            } else {
+                sm[lidx(wb, k)] = y;   // re-mapped below
+            }
+        };
+        rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load, store);
         __syncthreads();
     }
     if constexpr (VAR == VAR_4B) {

@@ -56,27 +56,18 @@ __device__ __forceinline__ void rag_fft(const RJob& jb, int m, cx<T>* bufA, cx<T
     for (int p = 0; p < jb.npass; ++p) {
         const int R = jb.rad[p];
         const int NS = jb.ns[p];
-        const int NR = m / R;
         const bool last = p == jb.npass - 1;
         cx<T>* rd = (p & 1) ? bufA : bufB;   // written by pass p-1
         cx<T>* wr = (p & 1) ? bufB : bufA;
-        for (int j = tid; j < NR; j += RAG_THREADS) {
-            cx<T> v[16];
-            for (int r = 0; r < R; ++r) {
-                const int n = j + r * NR;
-                v[r] = p == 0 ? src(n) : rd[n];
-            }
-            if (NS > 1) {
-                const cx<T>* w = tw + jb.twoff[p] + (j % NS) * (R - 1);
-                for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], w[r - 1]);
-            }
-            dft_rt<DIR>(R, v);
-            if (!last) {
-                const int base = (j / NS) * NS * R + (j % NS);
-                for (int r = 0; r < R; ++r) wr[base + r * NS] = v[r];
-            } else {
-                for (int r = 0; r < R; ++r) sink(j + r * NS, v[r]);
-            }
+        auto load = [&](int n) -> cx<T> { return p == 0 ? src(n) : rd[n]; };
+        auto store = [&](int k, cx<T> v) {
+            if (last) sink(k, v);
+            else wr[k] = v;
+        };
+        if (R == 1) {
+            for (int j = tid; j < m; j += RAG_THREADS) store(j, load(j));
+        } else {
+            rt_pass_any<DIR>(R, tid, RAG_THREADS, m, NS, tw + jb.twoff[p], load, store);
         }
         __syncthreads();
     }

@@ -128,14 +128,59 @@ struct AxisJob {
     int in_role, out_role;
     int64_t in_off, out_off;   // bytes
     double scale;
+    int strategy = AFFT_STRATEGY_AUTO;   // forced route (1-D plans)
+    int64_t split = 0;                   // forced four-step n1
 };
 
 static int emit_axis(Plan& p, int di, const AxisJob& j, std::string& algo);
+
+const void* mfma_kernel(int n, int dir, int* tpb);
+
+// MFMA form (AFFT_FLAG_MFMA): complex64 rows of N = 256 or 4096.
+static int emit_mfma(Plan& p, int di, const AxisJob& j, std::string& algo) {
+    const int dir = di ? 1 : -1;
+    int tpb = 1;
+    const void* fn = mfma_kernel((int)j.n, dir, &tpb);
+    const KernelInfo* k = find_kernel(0, (int)j.n, VAR_ROW, dir);
+    if (p.desc.precision != 0 || !fn || !k || j.in_nd != 1 || j.out_nd != 1)
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "the MFMA form covers complex64 rows of 256 or 4096 points");
+    KArgs a{};
+    int rc = schedule_twiddles(p, k, &a.tw);
+    if (rc) return rc;
+    a.count = j.count;
+    a.in_cd = j.in_cd;
+    a.out_cd = j.out_cd;
+    a.scale = j.scale;
+    Step s;
+    s.fn = fn;
+    s.grid = dim3((unsigned)((j.count + tpb - 1) / tpb), 1, 1);
+    s.block = dim3(256, 1, 1);
+    s.arg.resize(sizeof(KArgs));
+    std::memcpy(s.arg.data(), &a, sizeof(KArgs));
+    s.in_field = (int)offsetof(KArgs, in);
+    s.out_field = (int)offsetof(KArgs, out);
+    s.in_role = j.in_role;
+    s.out_role = j.out_role;
+    s.in_off = j.in_off;
+    s.out_off = j.out_off;
+    s.name = "mfma" + std::to_string(j.n);
+    const int es = 8;
+    for (int64_t y = 0; y < j.slices; ++y) {
+        Step t = s;
+        t.in_off += y * j.in_bd * es;
+        t.out_off += y * j.out_bd * es;
+        p.steps[di].push_back(t);
+    }
+    if (!algo.empty()) algo += "+";
+    algo += "mfma" + std::to_string(j.n) + "r";
+    return AFFT_OK;
+}
 
 static int emit_single(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int prec = p.desc.precision;
     const int dir = di ? 1 : -1;
     const int es = prec ? 16 : 8;
+    if (p.desc.flags & AFFT_FLAG_MFMA) return emit_mfma(p, di, j, algo);
     int var;
     int64_t count = j.count, slices = j.slices;
     int64_t in_cd = j.in_cd, out_cd = j.out_cd, in_bd = j.in_bd, out_bd = j.out_bd;
@@ -190,8 +235,15 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
     if (j.in_nd != 1 || j.out_nd != 1 || !is_pow2(j.n))
         return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step needs a unit-stride power-of-two axis");
     const int lg = ilog2(j.n);
-    const int64_t n1 = 1LL << (lg / 2), n2 = j.n / n1;
-    if (n2 > max_single_pass(prec)) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "transform too large");
+    int64_t n1 = 1LL << (lg / 2);
+    if (j.split) {
+        if (!is_pow2(j.split) || j.split < 2 || j.split >= j.n)
+            return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step split must be a power of two below n");
+        n1 = j.split;
+    }
+    const int64_t n2 = j.n / n1;
+    if (n2 > max_single_pass(prec) || n1 > max_single_pass(prec))
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "transform too large for this split");
     const KernelInfo* ka = find_kernel(prec, (int)n1, VAR_4A, dir);
     const KernelInfo* kb = find_kernel(prec, (int)n2, VAR_4B, dir);
     if (!ka || !kb) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "missing four-step kernel");
@@ -266,7 +318,7 @@ std::vector<int> smooth_schedule(int64_t n) {
     return r;
 }
 
-static int gen_cap(int prec) { return prec ? 4096 : 8192; }
+int gen_cap(int prec) { return prec ? 4096 : 8192; }
 
 int64_t next_pow2(int64_t n) {
     int64_t m = 1;
@@ -401,6 +453,11 @@ static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& alg
     int64_t n1 = 0;
     for (int64_t d = 1; d * d <= j.n; ++d)
         if (j.n % d == 0 && d <= cap && j.n / d <= cap) n1 = d;
+    if (j.split) {
+        if (j.split < 2 || j.n % j.split || j.split > cap || j.n / j.split > cap)
+            return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step split must divide n into LDS-sized factors");
+        n1 = j.split;
+    }
     if (!n1) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no four-step split for n=" + std::to_string(j.n));
     const int64_t n2 = j.n / n1;
     const int64_t total = j.count * j.slices;
@@ -458,15 +515,19 @@ static void host_fft_ld(std::vector<long double>& re, std::vector<long double>& 
         jj ^= bit;
         if (i < jj) { std::swap(re[i], re[jj]); std::swap(im[i], im[jj]); }
     }
+    // twiddles W_n^k, k < n/2, computed once
+    std::vector<long double> wr(n / 2 + 1), wi(n / 2 + 1);
+    for (size_t k = 0; k < n / 2; ++k) {
+        double c, s;
+        root_of_unity((int64_t)k, (int64_t)n, &c, &s);
+        wr[k] = c;
+        wi[k] = s;
+    }
     for (size_t len = 2; len <= n; len <<= 1) {
         const size_t half = len / 2, step = n / len;
-        for (size_t k = 0; k < half; ++k) {
-            double wr, wi;
-            root_of_unity((int64_t)(k * step), (int64_t)n, &wr, &wi);
-            // refine the twiddle to long double via the double value (exact
-            // inputs to 1 ulp of double; adequate for a double filter)
-            const long double cr = wr, ci = wi;
-            for (size_t b = 0; b < n; b += len) {
+        for (size_t b = 0; b < n; b += len) {
+            for (size_t k = 0; k < half; ++k) {
+                const long double cr = wr[k * step], ci = wi[k * step];
                 const size_t u = b + k, v = b + k + half;
                 const long double xr = re[v] * cr - im[v] * ci, xi = re[v] * ci + im[v] * cr;
                 re[v] = re[u] - xr; im[v] = im[u] - xi;
@@ -476,9 +537,6 @@ static void host_fft_ld(std::vector<long double>& re, std::vector<long double>& 
     }
 }
 
-// chirp[k] = exp(-iπ k²/N); with exact reduction k² mod 2N in integers
-// unless the reference's unreduced formula is requested
-// (internal/kernels/bluestein.go:20-32).
 int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt) {
     const bool refchirp = p.desc.flags & AFFT_FLAG_REF_CHIRP;
     std::vector<double> ch(2 * n);
@@ -709,8 +767,9 @@ static int emit_via_gather(Plan& p, int di, const AxisJob& j, std::string& algo)
     return AFFT_OK;
 }
 
-static bool needs_rows(const Plan& p, int64_t n) {
+static bool needs_rows(const Plan& p, int64_t n, int strategy) {
     const int prec = p.desc.precision;
+    if (strategy == AFFT_STRATEGY_SIXSTEP || strategy == AFFT_STRATEGY_BLUESTEIN) return true;
     if ((n & (n - 1)) == 0) return n > max_single_pass(prec);
     if (!smooth_schedule(n).empty()) return n > gen_cap(prec);
     return true;  // Bluestein
@@ -718,6 +777,25 @@ static bool needs_rows(const Plan& p, int64_t n) {
 
 static int emit_axis_direct(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int prec = p.desc.precision;
+    const bool smooth = !smooth_schedule(j.n).empty();
+    switch (j.strategy) {
+    case AFFT_STRATEGY_AUTO: break;
+    case AFFT_STRATEGY_STOCKHAM:
+        if (is_pow2(j.n) && j.n <= max_single_pass(prec)) return emit_single(p, di, j, algo);
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "Stockham strategy needs a power of two held in LDS");
+    case AFFT_STRATEGY_MIXED:
+        if (smooth && j.n <= gen_cap(prec)) return emit_gen_single(p, di, j, algo);
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "mixed-radix strategy needs a 7-smooth size held in LDS");
+    case AFFT_STRATEGY_SIXSTEP:
+        if (is_pow2(j.n)) return emit_fourstep(p, di, j, algo);
+        if (smooth) return emit_gen_fourstep(p, di, j, algo);
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step strategy needs a 7-smooth size");
+    case AFFT_STRATEGY_BLUESTEIN:
+        if (j.n >= 2) return emit_bluestein(p, di, j, algo);
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "Bluestein needs n >= 2");
+    default:
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "unknown strategy");
+    }
     if (is_pow2(j.n) && j.n <= max_single_pass(prec)) return emit_single(p, di, j, algo);
     if (is_pow2(j.n)) return emit_fourstep(p, di, j, algo);
     if (!smooth_schedule(j.n).empty()) {
@@ -741,7 +819,7 @@ static int emit_axis(Plan& p, int di, const AxisJob& j, std::string& algo) {
         algo += "copy";
         return AFFT_OK;
     }
-    if (needs_rows(p, j.n) && (j.in_nd != 1 || j.out_nd != 1)) return emit_via_gather(p, di, j, algo);
+    if (needs_rows(p, j.n, j.strategy) && (j.in_nd != 1 || j.out_nd != 1)) return emit_via_gather(p, di, j, algo);
     return emit_axis_direct(p, di, j, algo);
 }
 
@@ -907,6 +985,35 @@ static int emit_real_rows(Plan& p, int di, int64_t n, int64_t rows, int64_t slic
     return AFFT_OK;
 }
 
+// Planner's choice for AFFT_STRATEGY_AUTO on a 1-D axis: wisdom for this
+// size first, then the context's cost model, else the built-in routing
+// (cf. planner.go:43-78: registry, wisdom, heuristic).
+void choose_strategy(Plan& p, AxisJob& j) {
+    if (j.in_nd != 1 || j.out_nd != 1) return;
+    Context* c = p.ctx;
+    std::lock_guard<std::mutex> lock(c->mu);
+    auto it = c->wisdom.find(std::make_tuple(j.n, p.desc.precision, c->features()));
+    Candidate pick{0, 0};
+    if (it != c->wisdom.end()) {
+        if (parse_strategy(it->second.algorithm, &pick)) {
+            j.strategy = pick.strategy;
+            j.split = pick.split;
+        }
+        return;
+    }
+    if (!c->model) return;
+    std::vector<Candidate> cand = plan_candidates(p.desc);
+    if (cand.size() < 2) return;
+    std::vector<float> x(cand.size() * NFEAT), y(cand.size());
+    for (size_t i = 0; i < cand.size(); ++i) plan_features(p.desc, cand[i], &x[i * NFEAT]);
+    if (mlp_predict_host(c->model, x.data(), (int64_t)cand.size(), y.data()) != AFFT_OK) return;
+    size_t best = 0;
+    for (size_t i = 1; i < cand.size(); ++i)
+        if (y[i] < y[best]) best = i;
+    j.strategy = cand[best].strategy;
+    j.split = cand[best].split;
+}
+
 int build_plan(Plan& p) {
     afft_plan_desc& d = p.desc;
     if (d.precision != AFFT_COMPLEX64 && d.precision != AFFT_COMPLEX128)
@@ -950,6 +1057,9 @@ int build_plan(Plan& p) {
                 j.slices = 1;
                 j.in_cd = id; j.in_nd = is;
                 j.out_cd = od; j.out_nd = os;
+                j.strategy = d.strategy;
+                j.split = d.split;
+                if (!j.strategy) choose_strategy(p, j);
                 int rc = emit_axis(p, di, j, algo);
                 if (rc) return rc;
             } else {

@@ -31,6 +31,20 @@ const KernelInfo* find_kernel(int prec, int n, int var, int dir) {
     return nullptr;
 }
 
+uint64_t Context::features() const {
+    // "gfx950" -> 0x950: the architecture stands in for the reference's CPU
+    // feature bitmask in wisdom keys
+    uint64_t f = 0;
+    const char* a = props.gcnArchName;
+    if (a[0] == 'g' && a[1] == 'f' && a[2] == 'x')
+        for (const char* q = a + 3; *q && *q != ':'; ++q) {
+            int v = (*q >= '0' && *q <= '9') ? *q - '0' : (*q >= 'a' && *q <= 'f') ? *q - 'a' + 10 : -1;
+            if (v < 0) break;
+            f = f * 16 + (uint64_t)v;
+        }
+    return f;
+}
+
 int max_single_pass(int prec) {
     int m = 1;
     for (const KernelInfo& k : registry())

@@ -62,6 +62,31 @@ def traffic_bytes(n, batch):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
+def mfma_leg(torch, A, ctx, stream, x, steps, n, batch):
+    """The same headline workload through the MFMA form (FP16 hi/lo radix-16
+    products, FP32 accumulation): time and accuracy beside the plain path."""
+    if n not in (256, 4096):
+        return None
+    plan = A.Plan.from_desc(ctx, (n,), precision=A.COMPLEX64, batch=batch, flags=A.FLAG_MFMA)
+    y = torch.empty_like(x)
+    sh = stream.cuda_stream
+    plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    rows = np.linspace(0, batch - 1, 16).astype(np.int64)
+    ref = np.fft.fft(x[rows].cpu().numpy().astype(np.complex128), axis=1)
+    rel = float(np.linalg.norm(y[rows].cpu().numpy() - ref) / np.linalg.norm(ref))
+    plan.close()
+    return {"algorithm": f"hip_mfma{n}r", "ms_per_step": round(ms, 5),
+            "gflops": round(5.0 * n * np.log2(n) * batch / (ms * 1e-3) / 1e9, 1), "rel_l2_vs_fp64": rel}
+
+
 def ragged_leg(torch, A, ctx, stream, steps):
     """Mixed-size workload: 4096 complex64 transforms, lengths log-uniform in
     [16, 65536] (60% powers of two, 30% 7-smooth, 10% primes via Bluestein),
@@ -238,6 +263,9 @@ def main():
                      "device_copy_gbs": round(copy_gbs, 1), "frac_of_device_copy": round(gbs / copy_gbs, 4)},
     }
     plan.close()
+    out["mfma"] = mfma_leg(torch, A, ctx, stream, x, args.steps, n, batch)
+    if out["mfma"]:
+        out["mfma"]["speed_vs_plain"] = round(ms / out["mfma"]["ms_per_step"], 3)
     if not args.no_ragged:
         out["ragged"] = ragged_leg(torch, A, ctx, stream, max(3, args.steps // 5))
     ctx.close()

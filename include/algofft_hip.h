@@ -95,7 +95,19 @@ typedef struct {
     int64_t batch;       /* >= 1 */
     int64_t istride, idist, ostride, odist;
     int flags;
+    int strategy;        /* afft_strategy for 1-D plans, 0 = planner's choice */
+    int64_t split;       /* four-step first factor n1 (0 = balanced) */
 } afft_plan_desc;
+
+/* Forced kernel strategy (cf. PlanOptions.Strategy, plan_options.go:51-53,
+ * and the KernelStrategy values of internal/fftypes/strategy.go:4-14). */
+typedef enum {
+    AFFT_STRATEGY_AUTO = 0,      /* KernelAuto: wisdom, then cost model, then heuristic */
+    AFFT_STRATEGY_STOCKHAM = 2,  /* KernelStockham: one compile-time LDS pass set */
+    AFFT_STRATEGY_SIXSTEP = 3,   /* KernelSixStep: four-step with first factor `split` */
+    AFFT_STRATEGY_BLUESTEIN = 5, /* KernelBluestein */
+    AFFT_STRATEGY_MIXED = 16     /* runtime-size mixed-radix LDS kernel */
+} afft_strategy;
 
 typedef struct afft_context_s* afft_context;
 typedef struct afft_buffer_s* afft_buffer;
@@ -189,6 +201,65 @@ int afft_ragged_exec_graph(afft_ragged r, int direction, void* dst, const void* 
 /* Every job through the plan of its own size: one plan execution per job. */
 int afft_ragged_exec_per_size(afft_ragged r, int direction, void* dst, const void* src, void* stream);
 int afft_ragged_destroy(afft_ragged r);
+
+/* ---- planning: candidates, autotuner, wisdom -------------------------------
+ * The autotuner replaces MeasureAndSelect (internal/fft/measure.go:89-228);
+ * wisdom keeps the reference's text format "size:precision:features:
+ * algorithm:timestamp" (internal/planner/wisdom.go:96-205), with the GPU
+ * architecture number (0x950 for gfx950) in the features field. */
+/* Candidate strategies of a 1-D C2C shape; returns their count (arrays may
+ * be NULL to query it). */
+int afft_plan_candidates(const afft_plan_desc* desc, int* strategies, int64_t* splits, int cap);
+/* The 32 cost-model features of (shape, candidate). */
+int afft_plan_features(const afft_plan_desc* desc, int strategy, int64_t split, float* out32);
+/* Average milliseconds of one forward execution (device buffers). */
+int afft_tune_time(afft_context ctx, const afft_plan_desc* desc, int warmup, int iters, double* ms);
+/* Times every candidate (times[i] = ms of candidate i of
+ * afft_plan_candidates, -1 if it could not run) and stores the winner as
+ * wisdom in the context. */
+int afft_tune_plan(afft_context ctx, const afft_plan_desc* desc, int warmup, int iters, int* best_strategy,
+                   int64_t* best_split, double* best_ms, double* times, int cap);
+int afft_wisdom_export(afft_context ctx, char* buf, int64_t cap, int64_t* needed);  /* Wisdom.Export */
+int afft_wisdom_import(afft_context ctx, const char* text);                          /* Wisdom.Import */
+int afft_wisdom_clear(afft_context ctx);
+
+/* ---- learned cost model -----------------------------------------------------
+ * A 32->64->64->1 MLP predicting log runtime of (shape, candidate), trained
+ * and evaluated by the library's own HIP kernels.  No counterpart in the
+ * reference; it stands in front of the wisdom/timing planner. */
+typedef struct afft_mlp_s* afft_mlp;
+int afft_mlp_param_count(void);
+int afft_mlp_create(afft_context ctx, uint64_t seed, afft_mlp* out);
+int afft_mlp_set_normalization(afft_mlp m, const float* mean32, const float* std32, float ymean, float ystd);
+int afft_mlp_get_params(afft_mlp m, float* out);
+int afft_mlp_set_params(afft_mlp m, const float* in);
+int afft_mlp_loss_grad(afft_mlp m, const float* x, const float* t, int64_t n, float* loss, float* grad);
+int afft_mlp_train(afft_mlp m, const float* x, const float* t, int64_t n, int epochs, int batch, float lr,
+                   uint64_t seed, float* final_loss);
+int afft_mlp_predict(afft_mlp m, const float* x, int64_t n, float* y);
+int afft_mlp_save(afft_mlp m, const char* path);   /* weights + normalisation + FNV-1a checksum */
+int afft_mlp_load(afft_mlp m, const char* path);
+int afft_mlp_destroy(afft_mlp m);
+/* AUTO plans on this context ask the model when no wisdom entry matches. */
+int afft_context_set_cost_model(afft_context ctx, afft_mlp m);
+
+/* ---- convolution and correlation -------------------------------------------
+ * Convolve / Convolve128 (convolve.go:9-115) and CrossCorrelate /
+ * AutoCorrelate (correlate.go:6-79) on the device.  mode 0: linear
+ * convolution; mode 1: cross-correlation (b reversed and conjugated; output
+ * index k is lag k-(nb-1)).  Output length na+nb-1. */
+typedef struct afft_conv_s* afft_conv;
+int afft_conv_create(afft_context ctx, int precision, int64_t na, int64_t nb, int64_t batch, int mode,
+                     afft_conv* out);
+/* Device pointers: a = batch rows of na, b = batch rows of nb, dst = batch
+ * rows of na+nb-1 (complex elements); enqueued on stream (NULL = own). */
+int afft_conv_exec(afft_conv c, void* dst, const void* a, const void* b, void* stream);
+int afft_conv_destroy(afft_conv c);
+/* Host slices, one convolution, synchronised; the checks and errors of
+ * Convolve (nil -> ErrNilSlice, empty -> ErrInvalidLength, dst length !=
+ * na+nb-1 -> ErrLengthMismatch). */
+int afft_convolve_host(afft_context ctx, int precision, int mode, void* dst, int64_t dst_len, const void* a,
+                       int64_t na, const void* b, int64_t nb);
 
 #ifdef __cplusplus
 }

@@ -119,3 +119,53 @@ long or_next_pow2(long n) {
 #include "fft_impl.inc"
 #undef R
 #undef SFX
+
+/* ---- cost-model restatement ---------------------------------------------
+ * FP64 forward and backward pass of the 32 -> 64 -> 64 -> 1 ReLU MLP the
+ * library trains on the GPU (same parameter layout: W1[32][64], b1[64],
+ * W2[64][64], b2[64], W3[64], b3; loss = mean squared error).  It has no
+ * counterpart in the reference; it is the checker for the GPU gradients.
+ * X and T are the standardised inputs and targets. */
+int or_mlp_loss_grad(const double *P, const double *X, const double *T, long n, double *loss, double *G) {
+    enum { I = 32, H = 64 };
+    const long W1 = 0, B1 = I * H, W2 = B1 + H, B2 = W2 + H * H, W3 = B2 + H, B3 = W3 + H, NP = B3 + 1;
+    for (long p = 0; p < NP; p++) G[p] = 0;
+    double L = 0;
+    for (long s = 0; s < n; s++) {
+        const double *x = X + s * I;
+        double h1[H], h2[H], d2[H], d1[H];
+        for (int j = 0; j < H; j++) {
+            double a = P[B1 + j];
+            for (int i = 0; i < I; i++) a += x[i] * P[W1 + i * H + j];
+            h1[j] = a > 0 ? a : 0;
+        }
+        double y = P[B3];
+        for (int j = 0; j < H; j++) {
+            double a = P[B2 + j];
+            for (int i = 0; i < H; i++) a += h1[i] * P[W2 + i * H + j];
+            h2[j] = a > 0 ? a : 0;
+            y += h2[j] * P[W3 + j];
+        }
+        L += (y - T[s]) * (y - T[s]);
+        double dy = 2.0 * (y - T[s]) / (double)n;
+        G[B3] += dy;
+        for (int j = 0; j < H; j++) {
+            G[W3 + j] += h2[j] * dy;
+            d2[j] = h2[j] > 0 ? dy * P[W3 + j] : 0;
+            G[B2 + j] += d2[j];
+        }
+        for (int i = 0; i < H; i++) {
+            double a = 0;
+            for (int j = 0; j < H; j++) {
+                G[W2 + i * H + j] += h1[i] * d2[j];
+                a += P[W2 + i * H + j] * d2[j];
+            }
+            d1[i] = h1[i] > 0 ? a : 0;
+            G[B1 + i] += d1[i];
+        }
+        for (int k = 0; k < I; k++)
+            for (int i = 0; i < H; i++) G[W1 + k * H + i] += x[k] * d1[i];
+    }
+    *loss = L / (double)n;
+    return 0;
+}

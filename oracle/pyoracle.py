@@ -176,3 +176,19 @@ def rng_u01(seed, count, offset=0):
 def set_fault(f):
     """Plant a known mistake in the oracle (0 = none); see afft_oracle.c."""
     lib().or_set_fault(int(f))
+
+
+def mlp_loss_grad(params, X, T):
+    """FP64 loss and gradient of the cost-model MLP (checker for the GPU
+    training kernels); X and T already standardised."""
+    P = np.ascontiguousarray(params, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    G = np.empty_like(P)
+    loss = ctypes.c_double()
+    L = lib()
+    L.or_mlp_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+    rc = L.or_mlp_loss_grad(_p(P), _p(X), _p(T), len(T), ctypes.addressof(loss), _p(G))
+    assert rc == 0
+    return loss.value, G

@@ -1,0 +1,53 @@
+"""MFMA form (AFFT_FLAG_MFMA): radix-16 passes as FP16 hi/lo matrix products
+with FP32 accumulation, checked against the FP64 oracle at the complex64
+budget rel L2 <= 2e-6*log2(N), on uniform data and on the precision workload
+(8 tones, amplitudes 1e-3..1e1, noise)."""
+import numpy as np
+import pytest
+
+import algofft_hip as A
+from algofft_hip import synth
+import pyoracle as O
+from helpers import c64_tol, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(ctx, x, flags, inverse=False):
+    batch, n = x.shape
+    with A.Plan.from_desc(ctx, (n,), batch=batch, flags=flags) as p:
+        y = np.empty_like(x)
+        (p.inverse if inverse else p.forward)(y, x)
+        return y, p.describe()
+
+
+@pytest.mark.parametrize("n", [256, 4096])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_mfma_matches_oracle(ctx, n, inverse):
+    x = synth.uniform_complex(n, (64, n), np.complex64)
+    y, algo = _run(ctx, x, A.FLAG_MFMA, inverse)
+    assert algo.startswith("hip_mfma"), algo
+    ref = O.fft_batch(x.astype(np.complex128), inverse=inverse)
+    err = rel_l2(y, ref)
+    plain, _ = _run(ctx, x, 0, inverse)
+    print(f"n={n} inverse={inverse} mfma relL2={err:.3e} plain relL2={rel_l2(plain, ref):.3e}")
+    assert err <= c64_tol(n)
+
+
+@pytest.mark.parametrize("n", [256, 4096])
+def test_mfma_on_high_dynamic_range_tones(ctx, n):
+    x, bins, amps = synth.sinusoids(77, n, 32)
+    y, _ = _run(ctx, x, A.FLAG_MFMA)
+    ref = O.fft_batch(x.astype(np.complex128))
+    assert rel_l2(y, ref) <= c64_tol(n)
+    # the strongest tone of every row lands in its bin
+    for b in range(32):
+        j = int(np.argmax(amps[b]))
+        assert abs(y[b, bins[b, j]]) >= 0.5 * n * amps[b, j]
+
+
+def test_mfma_unsupported_shapes_are_reported(ctx):
+    with pytest.raises(A.ErrNotImplemented):
+        A.Plan.from_desc(ctx, (1024,), flags=A.FLAG_MFMA)
+    with pytest.raises(A.ErrNotImplemented):
+        A.Plan.from_desc(ctx, (4096,), precision=A.COMPLEX128, flags=A.FLAG_MFMA)
