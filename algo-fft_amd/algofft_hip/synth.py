@@ -30,7 +30,7 @@ def u01(seed, count, offset=0):
 
 
 def uniform_complex(seed, shape, dtype=np.complex64):
-    """i.i.d. U[-1, 1) real and imaginary parts (configs 1, 3, 5)."""
+    """i.i.d. U[-1, 1) real and imaginary parts."""
     n = int(np.prod(shape))
     u = u01(seed, 2 * n) * 2.0 - 1.0
     out = (u[0::2] + 1j * u[1::2]).astype(dtype)

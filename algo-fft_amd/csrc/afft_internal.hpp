@@ -118,7 +118,10 @@ struct BlueTables {
     const void* filt[2] = {nullptr, nullptr};    // [fwd, inv], m entries, 1/m folded in
     const void* post[2] = {nullptr, nullptr};    // [fwd, inv]: chirp, conj(chirp)*inv_scale
 };
-int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt);
+// tr_m1 > 0 stores the filter spectra row-major as [k1][k2], k = k1 + tr_m1*k2
+int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt, int64_t tr_m1 = 0);
+// W_L^i = hi[i >> shift] * lo[i & (2^shift - 1)], both FP64-generated
+int split_twiddles(Plan& p, int prec, int64_t L, int* shift, const void** lo, const void** hi);
 int max_single_pass(int prec);
 int gen_cap(int prec);
 

@@ -141,9 +141,22 @@ __device__ __forceinline__ void dft4(cx<T>* v) {
 
 // Odd prime radix by the conjugate-pair form:
 //   y_k = a0 + Σ_j c_jk b_j  +  DIR*i Σ_j s_jk d_j,   b_j = a_j + a_{R-j}, d_j = a_j - a_{R-j}
+// cos / sin of 2*pi*i/R for i < R, built at compile time
+template <int R>
+struct RootTable {
+    double c[R], s[R];
+    constexpr RootTable() : c(), s() {
+        for (int i = 0; i < R; ++i) {
+            c[i] = cos2pi(i, R);
+            s[i] = sin2pi(i, R);
+        }
+    }
+};
+
 template <int R, int DIR, int S, typename T>
 __device__ __forceinline__ void dft_odd(cx<T>* v) {
     constexpr int H = (R - 1) / 2;
+    constexpr RootTable<R> tab{};
     cx<T> b[H], d[H];
     cx<T> a0 = v[0];
     cx<T> y0 = a0;
@@ -158,8 +171,8 @@ __device__ __forceinline__ void dft_odd(cx<T>* v) {
         T rx = a0.x, ry = a0.y, ix = 0, iy = 0;
 #pragma unroll
         for (int j = 1; j <= H; ++j) {
-            const T c = T(cos2pi((long)j * k, R));
-            const T s = T(sin2pi((long)j * k, R));
+            const T c = T(tab.c[(j * k) % R]);
+            const T s = T(tab.s[(j * k) % R]);
             rx += c * b[j - 1].x;
             ry += c * b[j - 1].y;
             ix += s * d[j - 1].x;

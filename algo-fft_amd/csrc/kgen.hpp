@@ -85,7 +85,7 @@ template <typename T, int VAR, int DIR>
 __global__ void __launch_bounds__(GEN_THREADS) k_gen(GArgs g) {
     extern __shared__ unsigned char smraw[];
     cx<T>* sm = (cx<T>*)smraw;
-    const KArgs& a = g.k;
+    const KArgs a = g.k;
     const int N = g.n;
     const int TPT = g.tpt;
     const int G = GEN_THREADS / TPT;
@@ -112,15 +112,12 @@ __global__ void __launch_bounds__(GEN_THREADS) k_gen(GArgs g) {
         const bool last = p == g.npass - 1;
         const cx<T>* twp = tw + g.twoff[p];
         const int rb = (p - 1) & 1, wb = p & 1;
-        auto load = [&](int n) -> cx<T> {
-            if (p == 0) return act ? src_g[(long long)n * ind] : mk<T>(0, 0);
-            return sm[lidx(rb, n)];
-        };
-        auto store = [&](int k, cx<T> y) {
-            if (!last) {
-                sm[lidx(wb, k)] = y;
-                return;
-            }
+        // separate lambdas (and rt_pass instances) for global and LDS sides,
+        // so no store or load has to choose its address space at run time
+        auto load_g = [&](int n) -> cx<T> { return act ? src_g[(long long)n * ind] : mk<T>(0, 0); };
+        auto load_s = [&](int n) -> cx<T> { return sm[lidx(rb, n)]; };
+        auto store_s = [&](int k, cx<T> y) { sm[lidx(wb, k)] = y; };
+        auto store_o = [&](int k, cx<T> y) {
             if constexpr (VAR == VAR_4A) {
                 const long long m = (long long)k * (cg + a.tw4_c0);
                 y = twmul<DIR>(y, tw4<T>(a, m));
@@ -134,7 +131,13 @@ __global__ void __launch_bounds__(GEN_THREADS) k_gen(GArgs g) {
                 sm[lidx(wb, k)] = y;   // re-mapped below
             }
         };
-        rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load, store);
+        if (p == 0) {
+            if (last) rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load_g, store_o);
+            else rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load_g, store_s);
+        } else {
+            if (last) rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load_s, store_o);
+            else rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load_s, store_s);
+        }
         __syncthreads();
     }
     if constexpr (VAR == VAR_4B) {

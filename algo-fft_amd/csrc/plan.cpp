@@ -71,7 +71,7 @@ static int schedule_twiddles(Plan& p, const KernelInfo* k, const void** dev) {
 }
 
 // Two-level table for W_L^m = hi[m >> sh] * lo[m & (2^sh - 1)].
-static int split_twiddles(Plan& p, int prec, int64_t L, int* shift, const void** lo, const void** hi) {
+int split_twiddles(Plan& p, int prec, int64_t L, int* shift, const void** lo, const void** hi) {
     const int sh = (ilog2(L) + 1) / 2;
     const int64_t nlo = 1LL << sh;
     const int64_t nhi = (L + nlo - 1) / nlo;
@@ -537,7 +537,7 @@ static void host_fft_ld(std::vector<long double>& re, std::vector<long double>& 
     }
 }
 
-int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt) {
+int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt, int64_t tr_m1) {
     const bool refchirp = p.desc.flags & AFFT_FLAG_REF_CHIRP;
     std::vector<double> ch(2 * n);
     for (int64_t k = 0; k < n; ++k) {
@@ -564,9 +564,12 @@ int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt)
         }
         host_fft_ld(re, im);
         filt[d].resize(2 * m);
+        // natural order, or row-major [k1][k2] of k = k1 + tr_m1 * k2
+        const int64_t m2 = tr_m1 > 0 ? m / tr_m1 : 0;
         for (int64_t k = 0; k < m; ++k) {
-            filt[d][2 * k] = (double)(re[k] / (long double)m);
-            filt[d][2 * k + 1] = (double)(im[k] / (long double)m);
+            const int64_t at = tr_m1 > 0 ? (k % tr_m1) * m2 + k / tr_m1 : k;
+            filt[d][2 * at] = (double)(re[k] / (long double)m);
+            filt[d][2 * at + 1] = (double)(im[k] / (long double)m);
         }
     }
     std::vector<double> post[2];

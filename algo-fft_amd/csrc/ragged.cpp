@@ -2,10 +2,12 @@
 // lengths, packed back to back (job i at offset sum of the earlier lengths).
 //
 // Three ways to run the same batch:
-//   afft_ragged_exec        one launch of the persistent work-queue kernel
-//                           (plus the counter reset kernel)
-//   afft_ragged_exec_graph  the same pair captured once in a hipGraph and
-//                           replayed
+//   afft_ragged_exec        the persistent work-queue kernel, one launch per
+//                           phase (whole jobs and column tasks; row tasks;
+//                           the Bluestein jobs' second column tasks), after
+//                           one counter-reset kernel
+//   afft_ragged_exec_graph  the same launches captured once in a hipGraph
+//                           and replayed as one graph launch
 //   afft_ragged_exec_per_size  every job through the plan of its own size,
 //                           one plan execution per job: the launch-per-size
 //                           form the persistent kernel is measured against
@@ -20,6 +22,8 @@
 
 using namespace afft;
 
+constexpr int RAG_PHASES = 3;
+
 struct afft_ragged_s {
     afft_context ctx = nullptr;
     int prec = 0;
@@ -29,7 +33,10 @@ struct afft_ragged_s {
     int64_t total = 0;
     Plan tables;                  // owns the device tables
     RJob* jobs_dev = nullptr;
+    RTask* tasks_dev = nullptr;
     int njobs = 0;
+    int ntask[RAG_PHASES] = {0, 0, 0};
+    int task_off[RAG_PHASES] = {0, 0, 0};
     int* counter = nullptr;
     void* scratch = nullptr;
     int64_t scratch_pts = 0;
@@ -46,14 +53,30 @@ afft_ragged_s::~afft_ragged_s() {
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto& kv : per_size) afft_plan_destroy(kv.second);
     if (jobs_dev) (void)hipFree(jobs_dev);
+    if (tasks_dev) (void)hipFree(tasks_dev);
     if (counter) (void)hipFree(counter);
     if (scratch) (void)hipFree(scratch);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-static int schedule_into(Plan& tab, int prec, int64_t m, RJob& j, std::map<int64_t, const void*>& cache) {
+// Radix schedule and per-pass twiddles of length m (m <= RAG_WHOLE), shared
+// between jobs through `cache`.
+static int schedule_into(Plan& tab, int prec, int64_t m, RSched& j, std::map<int64_t, RSched>& cache) {
+    auto it = cache.find(m);
+    if (it != cache.end()) {
+        j = it->second;
+        return AFFT_OK;
+    }
+    std::memset(&j, 0, sizeof(j));
+    if (m == 1) {
+        j.npass = 1;
+        j.rad[0] = 1;
+        j.ns[0] = 1;
+        cache[m] = j;
+        return AFFT_OK;
+    }
     std::vector<int> rad = smooth_schedule(m);
-    if (rad.empty() || (int)rad.size() > RAG_MAXPASS)
+    if (rad.empty() || (int)rad.size() > RAG_SPASS)
         return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no radix schedule for " + std::to_string(m));
     j.npass = (int)rad.size();
     int64_t ns = 1;
@@ -75,31 +98,50 @@ static int schedule_into(Plan& tab, int prec, int64_t m, RJob& j, std::map<int64
         }
         ns *= rad[i];
     }
-    auto it = cache.find(m);
-    if (it != cache.end()) {
-        j.tw = it->second;
-        return AFFT_OK;
-    }
     int rc;
-    if (prec == 0) {
+    if (tw.empty()) {
+        j.tw = nullptr;
+    } else if (prec == 0) {
         std::vector<float> f(tw.begin(), tw.end());
-        rc = upload_table(tab, f.data(), f.size() * 4, &j.tw);
+        if ((rc = upload_table(tab, f.data(), f.size() * 4, &j.tw))) return rc;
     } else {
-        rc = upload_table(tab, tw.data(), tw.size() * 8, &j.tw);
+        if ((rc = upload_table(tab, tw.data(), tw.size() * 8, &j.tw))) return rc;
     }
-    if (rc) return rc;
-    cache[m] = j.tw;
+    cache[m] = j;
     return AFFT_OK;
+}
+
+// m = m1 * m2 with m1 <= m2 <= RAG_WHOLE and m1 as close to sqrt(m) as the
+// divisors allow; false when m has no such split.
+static bool split_len(int64_t m, int64_t* m1, int64_t* m2) {
+    for (int64_t a = (int64_t)std::sqrt((double)m) + 1; a >= 2; --a) {
+        if (a * a > m || m % a) continue;
+        if (m / a > RAG_WHOLE) return false;
+        *m1 = a;
+        *m2 = m / a;
+        return true;
+    }
+    return false;
+}
+
+static int pow2_floor(int64_t v) {
+    int g = 1;
+    while ((int64_t)g * 2 <= v) g *= 2;
+    return g;
 }
 
 static int ragged_build(afft_ragged_s& r) {
     const int es = r.prec ? 16 : 8;
     std::vector<RJob> jobs(r.sizes.size());
-    std::vector<double> cost(r.sizes.size());
-    std::map<int64_t, const void*> twcache;
-    std::map<int64_t, BlueTables> bcache;
+    std::map<int64_t, RSched> scache;
+    std::map<std::pair<int64_t, int64_t>, BlueTables> bcache;
+    struct TwSplit { int shift; const void* lo; const void* hi; };
+    std::map<int64_t, TwSplit> tcache;
+    struct Pending { RTask t; double cost; };
+    std::vector<Pending> ph[RAG_PHASES];
     int64_t off = 0;
     r.offs.resize(r.sizes.size());
+    r.scratch_pts = 0;
     for (size_t i = 0; i < r.sizes.size(); ++i) {
         const int64_t n = r.sizes[i];
         if (n < 1) return set_error(AFFT_ERR_INVALID_LENGTH, "job length must be >= 1");
@@ -110,75 +152,114 @@ static int ragged_build(afft_ragged_s& r) {
         r.offs[i] = off;
         off += n;
         j.n = (int)n;
-        const bool smooth = !smooth_schedule(n).empty() || n == 1;
-        if (n == 1) {
-            // a 1-point transform: one radix-1-free schedule is the copy; use
-            // m = 1 with a single trivial pass
-            j.m = 1;
-            j.npass = 1;
-            j.rad[0] = 1;
-            j.ns[0] = 1;
-            j.tw = nullptr;
-        } else if (smooth) {
-            j.m = (int)n;
-            int rc = schedule_into(r.tables, r.prec, n, j, twcache);
-            if (rc) return rc;
+        int64_t m = n, m1 = 0, m2 = 0;
+        bool blue = !(n == 1 || !smooth_schedule(n).empty());
+        if (!blue && m > RAG_WHOLE && !split_len(m, &m1, &m2)) blue = true;   // no usable split
+        if (blue) {
+            m = next_pow2(2 * n - 1);
+            if (m > RAG_WHOLE) split_len(m, &m1, &m2);
+        }
+        j.m = (int)m;
+        j.blue = blue;
+        const bool whole = m <= RAG_WHOLE;
+        int rc;
+        if (whole) {
+            if ((rc = schedule_into(r.tables, r.prec, m, j.s[0], scache))) return rc;
         } else {
-            j.blue = 1;
-            j.m = (int)next_pow2(2 * n - 1);
-            int rc = schedule_into(r.tables, r.prec, j.m, j, twcache);
-            if (rc) return rc;
-            auto it = bcache.find(n);
+            j.m1 = (int)m1;
+            j.m2 = (int)m2;
+            if ((rc = schedule_into(r.tables, r.prec, m1, j.s[0], scache))) return rc;
+            if ((rc = schedule_into(r.tables, r.prec, m2, j.s[1], scache))) return rc;
+            auto tt = tcache.find(m);
+            if (tt == tcache.end()) {
+                TwSplit ts{};
+                if ((rc = split_twiddles(r.tables, r.prec, m, &ts.shift, &ts.lo, &ts.hi))) return rc;
+                tt = tcache.emplace(m, ts).first;
+            }
+            j.shift = tt->second.shift;
+            j.twlo = tt->second.lo;
+            j.twhi = tt->second.hi;
+            j.g_col = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_HALF / (m1 + 1), m2}));
+            j.g_row = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_HALF / (m2 + 1), m1}));
+            j.scr_off = r.scratch_pts;
+            r.scratch_pts += m;
+        }
+        if (blue) {
+            const auto key = std::make_pair(n, whole ? (int64_t)0 : m1);
+            auto it = bcache.find(key);
             if (it == bcache.end()) {
                 BlueTables bt;
                 r.tables.desc.precision = r.prec;
                 r.tables.desc.flags = r.flags;
-                if ((rc = blue_tables(r.tables, n, j.m, 1.0, bt))) return rc;
-                it = bcache.emplace(n, bt).first;
+                if ((rc = blue_tables(r.tables, n, m, 1.0, bt, key.second))) return rc;
+                it = bcache.emplace(key, bt).first;
             }
             j.chirp = it->second.chirp;
             j.filt[0] = it->second.filt[0];
             j.filt[1] = it->second.filt[1];
         }
-        const double m = j.m;
-        cost[i] = (j.blue ? 2.0 : 1.0) * m * std::log2(std::max(2.0, m)) * (2 * m > RAG_LDS ? 3.0 : 1.0);
-        if (2 * j.m > RAG_LDS) r.scratch_pts = std::max<int64_t>(r.scratch_pts, j.m);
+        const int job = (int)i;
+        if (whole) {
+            const double c = (blue ? 2.0 : 1.0) * m * std::log2(std::max<double>(2.0, (double)m));
+            ph[0].push_back({{job, RT_WHOLE, 0, 1}, c});
+            continue;
+        }
+        // staging adds about one pass of LDS traffic to every task
+        const double ccol = (double)m1 * (std::log2((double)m1) + 2.0);
+        const double crow = (double)m2 * (std::log2((double)m2) + 2.0) * (blue ? 2.0 : 1.0);
+        for (int64_t c0 = 0; c0 < m2; c0 += j.g_col) {
+            const int cnt = (int)std::min<int64_t>(j.g_col, m2 - c0);
+            ph[0].push_back({{job, RT_COLS, (int)c0, cnt}, ccol * cnt});
+            if (blue) ph[2].push_back({{job, RT_COLS2, (int)c0, cnt}, ccol * cnt});
+        }
+        for (int64_t r0 = 0; r0 < m1; r0 += j.g_row) {
+            const int cnt = (int)std::min<int64_t>(j.g_row, m1 - r0);
+            ph[1].push_back({{job, RT_ROWS, (int)r0, cnt}, crow * cnt});
+        }
     }
     r.total = off;
-    // longest-processing-time order
-    std::vector<size_t> order(jobs.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-    std::vector<RJob> sorted;
-    sorted.reserve(jobs.size());
-    for (size_t i : order) sorted.push_back(jobs[i]);
-    r.njobs = (int)sorted.size();
-    AFFT_HIP(hipMalloc(&r.jobs_dev, std::max<size_t>(1, sorted.size()) * sizeof(RJob)));
-    if (!sorted.empty()) AFFT_HIP(hipMemcpy(r.jobs_dev, sorted.data(), sorted.size() * sizeof(RJob), hipMemcpyHostToDevice));
+    // longest-processing-time order within each phase
+    std::vector<RTask> tasks;
+    for (int p = 0; p < RAG_PHASES; ++p) {
+        std::stable_sort(ph[p].begin(), ph[p].end(), [](const Pending& a, const Pending& b) { return a.cost > b.cost; });
+        r.task_off[p] = (int)tasks.size();
+        r.ntask[p] = (int)ph[p].size();
+        for (auto& q : ph[p]) tasks.push_back(q.t);
+    }
+    r.njobs = (int)jobs.size();
+    AFFT_HIP(hipMalloc(&r.jobs_dev, std::max<size_t>(1, jobs.size()) * sizeof(RJob)));
+    AFFT_HIP(hipMemcpy(r.jobs_dev, jobs.data(), jobs.size() * sizeof(RJob), hipMemcpyHostToDevice));
+    AFFT_HIP(hipMalloc(&r.tasks_dev, std::max<size_t>(1, tasks.size()) * sizeof(RTask)));
+    if (!tasks.empty())
+        AFFT_HIP(hipMemcpy(r.tasks_dev, tasks.data(), tasks.size() * sizeof(RTask), hipMemcpyHostToDevice));
     AFFT_HIP(hipMalloc(&r.counter, 64));
-    // two blocks per CU: each block holds 64 KiB of LDS
-    r.grid = 2 * r.ctx->props.multiProcessorCount;
-    if (r.grid < 1) r.grid = 1;
-    const size_t sbytes = (size_t)r.grid * 2 * std::max<int64_t>(1, r.scratch_pts) * es;
-    AFFT_HIP(hipMalloc(&r.scratch, sbytes));
+    const void* fn = r.prec ? (const void*)&k_ragged<double> : (const void*)&k_ragged<float>;
+    int per_cu = 0;
+    AFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, RAG_THREADS, 0));
+    r.grid = std::max(1, per_cu) * std::max(1, r.ctx->props.multiProcessorCount);
+    AFFT_HIP(hipMalloc(&r.scratch, (size_t)std::max<int64_t>(1, r.scratch_pts) * es));
     return AFFT_OK;
 }
 
 static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, hipStream_t s) {
-    RagArgs a{};
-    a.in = src;
-    a.out = dst;
-    a.jobs = r.jobs_dev;
-    a.njobs = r.njobs;
-    a.dir = dir;
-    a.counter = r.counter;
-    a.scratch = r.scratch;
-    a.scratch_pts = std::max<int64_t>(1, r.scratch_pts);
     void* rargs[] = {&r.counter};
     AFFT_HIP(hipLaunchKernel((const void*)&k_rag_reset, dim3(1), dim3(64), rargs, 0, s));
-    void* args[] = {&a};
     const void* fn = r.prec ? (const void*)&k_ragged<double> : (const void*)&k_ragged<float>;
-    AFFT_HIP(hipLaunchKernel(fn, dim3(r.grid), dim3(RAG_THREADS), args, 0, s));
+    for (int p = 0; p < RAG_PHASES; ++p) {
+        if (r.ntask[p] == 0) continue;
+        RagArgs a{};
+        a.in = src;
+        a.out = dst;
+        a.jobs = r.jobs_dev;
+        a.tasks = r.tasks_dev + r.task_off[p];
+        a.ntasks = r.ntask[p];
+        a.dir = dir;
+        a.counter = r.counter + p;
+        a.scratch = r.scratch;
+        void* args[] = {&a};
+        const int grid = std::min(r.grid, r.ntask[p]);
+        AFFT_HIP(hipLaunchKernel(fn, dim3(grid), dim3(RAG_THREADS), args, 0, s));
+    }
     return AFFT_OK;
 }
 

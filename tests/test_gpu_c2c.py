@@ -61,8 +61,8 @@ def test_inverse_and_roundtrip(ctx, n, dtype):
     assert rel_l2(y, x) <= 2 * tol
 
 
-def test_config1_batch_equals_loop(ctx):
-    # config 1: N=1024, batch 256, contiguous; batch result == per-transform
+def test_batch_1024x256_equals_loop(ctx):
+    # N=1024, batch 256, contiguous: the batch result equals the per-transform
     # results (plan_batch_test.go:13-146)
     n, batch = 1024, 256
     x = synth.uniform_complex(1, (batch, n), np.complex64)
