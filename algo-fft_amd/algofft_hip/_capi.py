@@ -48,6 +48,8 @@ SIGNATURES = [
     ("afft_buffer_download", c_int, [c_void_p, c_void_p, c_i64]),
     ("afft_buffer_device_ptr", c_void_p, [c_void_p]),
     ("afft_buffer_destroy", c_int, [c_void_p]),
+    ("afft_host_alloc", c_int, [c_void_p, c_i64, ctypes.POINTER(c_void_p)]),
+    ("afft_host_free", c_int, [c_void_p]),
     ("afft_stream_create", c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
     ("afft_stream_synchronize", c_int, [c_void_p]),
     ("afft_stream_destroy", c_int, [c_void_p]),

@@ -102,10 +102,12 @@ struct Plan {
     void* stage_in = nullptr;
     void* stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
+    struct HostPipe* pipe = nullptr;   // chunked host-slice execution (runtime.cpp)
     std::string algo;
     std::mutex mu;
     ~Plan();
 };
+void destroy_host_pipe(HostPipe* h);
 
 int build_plan(Plan& p);
 

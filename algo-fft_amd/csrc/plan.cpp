@@ -26,6 +26,7 @@ Plan::~Plan() {
         if (t) (void)hipFree(t);
     if (stage_in) (void)hipFree(stage_in);
     if (stage_out) (void)hipFree(stage_out);
+    if (pipe) destroy_host_pipe(pipe);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -134,27 +135,31 @@ struct AxisJob {
 
 static int emit_axis(Plan& p, int di, const AxisJob& j, std::string& algo);
 
-const void* mfma_kernel(int n, int dir, int* tpb);
 
 // MFMA form (AFFT_FLAG_MFMA): complex64 rows of N = 256 or 4096.
+const void* mfma_kernel(int n, int var, int dir, int* per_block, int* threads);
+
+// MFMA form of a single-pass row transform (n = 256, 1024 or 4096).
 static int emit_mfma(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int dir = di ? 1 : -1;
-    int tpb = 1;
-    const void* fn = mfma_kernel((int)j.n, dir, &tpb);
+    int g = 1, threads = 256;
+    const void* fn = mfma_kernel((int)j.n, VAR_ROW, dir, &g, &threads);
     const KernelInfo* k = find_kernel(0, (int)j.n, VAR_ROW, dir);
     if (p.desc.precision != 0 || !fn || !k || j.in_nd != 1 || j.out_nd != 1)
-        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "the MFMA form covers complex64 rows of 256 or 4096 points");
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED,
+                         "the MFMA form covers complex64 rows of 256, 1024, 4096, 65536, 262144 or 1048576 points");
     KArgs a{};
     int rc = schedule_twiddles(p, k, &a.tw);
     if (rc) return rc;
     a.count = j.count;
     a.in_cd = j.in_cd;
     a.out_cd = j.out_cd;
+    a.in_nd = a.out_nd = 1;
     a.scale = j.scale;
     Step s;
     s.fn = fn;
-    s.grid = dim3((unsigned)((j.count + tpb - 1) / tpb), 1, 1);
-    s.block = dim3(256, 1, 1);
+    s.grid = dim3((unsigned)((j.count + g - 1) / g), 1, 1);
+    s.block = dim3((unsigned)threads, 1, 1);
     s.arg.resize(sizeof(KArgs));
     std::memcpy(s.arg.data(), &a, sizeof(KArgs));
     s.in_field = (int)offsetof(KArgs, in);
@@ -235,8 +240,17 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
     if (j.in_nd != 1 || j.out_nd != 1 || !is_pow2(j.n))
         return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step needs a unit-stride power-of-two axis");
     const int lg = ilog2(j.n);
+    const bool mf = p.desc.flags & AFFT_FLAG_MFMA;
     int64_t n1 = 1LL << (lg / 2);
-    if (j.split) {
+    if (mf) {
+        // the MFMA passes have lengths 256 and 1024
+        n1 = 0;
+        for (int64_t c1 : {256, 1024})
+            if (!n1 && (j.n == c1 * 256 || j.n == c1 * 1024) && (!j.split || j.split == c1)) n1 = c1;
+        if (!n1 || prec != 0)
+            return set_error(AFFT_ERR_NOT_IMPLEMENTED,
+                             "the MFMA form covers complex64 rows of 256, 1024, 4096, 65536, 262144 or 1048576 points");
+    } else if (j.split) {
         if (!is_pow2(j.split) || j.split < 2 || j.split >= j.n)
             return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step split must be a power of two below n");
         n1 = j.split;
@@ -247,6 +261,15 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const KernelInfo* ka = find_kernel(prec, (int)n1, VAR_4A, dir);
     const KernelInfo* kb = find_kernel(prec, (int)n2, VAR_4B, dir);
     if (!ka || !kb) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "missing four-step kernel");
+    // per-block transforms and launch shape of the two passes
+    int ga = ka->tpb, gb = kb->tpb, ta = ka->threads, tb = kb->threads;
+    const void* fa = ka->fn;
+    const void* fb = kb->fn;
+    if (mf) {
+        fa = mfma_kernel((int)n1, VAR_4A, dir, &ga, &ta);
+        fb = mfma_kernel((int)n2, VAR_4B, dir, &gb, &tb);
+        if (!fa || !fb) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "missing MFMA four-step kernel");
+    }
     // every (c, y) transform is one slice of the four-step passes
     const int64_t total = j.count * j.slices;
     const int64_t need = total * j.n * es;
@@ -279,13 +302,17 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
         KArgs aa = a, bb = b;
         aa.in_bd = j.in_cd;
         bb.out_bd = j.out_cd;
-        Step sa = make_step(ka, aa, (n2 + ka->tpb - 1) / ka->tpb, 1, "4stepA");
+        Step sa = make_step(ka, aa, (n2 + ga - 1) / ga, 1, "4stepA");
+        sa.fn = fa;
+        sa.block = dim3((unsigned)ta, 1, 1);
         sa.in_role = j.in_role;
         sa.in_off = j.in_off + y * j.in_bd * es;
         sa.out_role = ROLE_TMP0;
         sa.out_off = y * j.count * j.n * es;
         push_sliced(p.steps[di], sa, j.count, j.in_cd * es, j.n * es);
-        Step sb = make_step(kb, bb, (n1 + kb->tpb - 1) / kb->tpb, 1, "4stepB");
+        Step sb = make_step(kb, bb, (n1 + gb - 1) / gb, 1, "4stepB");
+        sb.fn = fb;
+        sb.block = dim3((unsigned)tb, 1, 1);
         sb.in_role = ROLE_TMP0;
         sb.in_off = y * j.count * j.n * es;
         sb.out_role = j.out_role;
@@ -293,7 +320,7 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
         push_sliced(p.steps[di], sb, j.count, j.n * es, j.out_cd * es);
     }
     if (!algo.empty()) algo += "+";
-    algo += "fourstep" + std::to_string(n1) + "x" + std::to_string(n2);
+    algo += (mf ? "mfma_fourstep" : "fourstep") + std::to_string(n1) + "x" + std::to_string(n2);
     return AFFT_OK;
 }
 
@@ -781,6 +808,8 @@ static bool needs_rows(const Plan& p, int64_t n, int strategy) {
 static int emit_axis_direct(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int prec = p.desc.precision;
     const bool smooth = !smooth_schedule(j.n).empty();
+    if ((p.desc.flags & AFFT_FLAG_MFMA) && !is_pow2(j.n))
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "the MFMA form covers powers of two only");
     switch (j.strategy) {
     case AFFT_STRATEGY_AUTO: break;
     case AFFT_STRATEGY_STOCKHAM:

@@ -1,5 +1,6 @@
 // C ABI of algofft_hip: errors, kernel registry, Backend / Context / Buffer /
 // Stream / PlanImpl entry points (the Go interfaces of gpu/backend.go:7-51).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -282,6 +283,23 @@ int afft_buffer_destroy(afft_buffer b) {
     return AFFT_OK;
 }
 
+int afft_host_alloc(afft_context ctx, int64_t bytes, void** out) {
+    if (!ctx || !out) return set_error(AFFT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (bytes < 1) return set_error(AFFT_ERR_INVALID_LENGTH, "bytes must be >= 1");
+    AFFT_HIP(hipSetDevice(ctx->device));
+    AFFT_HIP(hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault));
+    clear_error();
+    return AFFT_OK;
+}
+
+int afft_host_free(void* ptr) {
+    if (!ptr) return set_error(AFFT_ERR_NIL_SLICE, "nil pointer");
+    AFFT_HIP(hipHostFree(ptr));
+    clear_error();
+    return AFFT_OK;
+}
+
 int afft_stream_create(afft_context ctx, afft_stream* out) {
     if (!ctx || !out) return set_error(AFFT_ERR_INVALID_ARGUMENT, "NULL handle");
     *out = nullptr;
@@ -373,6 +391,161 @@ int afft_plan_io_len(afft_plan p, int direction, int64_t* in_len, int64_t* out_l
     return AFFT_OK;
 }
 
+// ---- host-slice execution ------------------------------------------------------
+// A batched 1-D plan whose host slices exceed kPipeMinBytes runs in chunks of
+// whole transforms on two streams: chunk k's upload, transform and download
+// are queued on stream k & 1, so one chunk's upload overlaps the previous
+// chunk's transform and download.  Pinned caller memory (afft_host_alloc or
+// any hipHostMalloc block) is transferred directly; pageable memory goes
+// through two pinned staging slots per direction, copied on the host while
+// the other slot's transfers run.
+constexpr size_t kPipeMinBytes = size_t(64) << 20;
+constexpr size_t kPipeChunkBytes = size_t(16) << 20;
+
+}  // extern "C"
+
+namespace afft {
+struct HostPipe {
+    int dir_index = -1;
+    int64_t chunk = 0;                 // transforms per full chunk
+    afft_plan full[2] = {nullptr, nullptr};
+    afft_plan tail = nullptr;
+    int64_t tail_batch = 0;
+    void* hin[2] = {nullptr, nullptr};
+    void* hout[2] = {nullptr, nullptr};
+    void* din[2] = {nullptr, nullptr};
+    void* dout[2] = {nullptr, nullptr};
+    hipStream_t s[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+void destroy_host_pipe(HostPipe* h) {
+    if (!h) return;
+    for (int i = 0; i < 2; ++i) {
+        if (h->s[i]) (void)hipStreamSynchronize(h->s[i]);
+        if (h->full[i]) afft_plan_destroy(h->full[i]);
+        if (h->hin[i]) (void)hipHostFree(h->hin[i]);
+        if (h->hout[i]) (void)hipHostFree(h->hout[i]);
+        if (h->din[i]) (void)hipFree(h->din[i]);
+        if (h->dout[i]) (void)hipFree(h->dout[i]);
+        if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
+        if (h->s[i]) (void)hipStreamDestroy(h->s[i]);
+    }
+    if (h->tail) afft_plan_destroy(h->tail);
+    delete h;
+}
+}  // namespace afft
+
+extern "C" {
+
+static bool host_pinned(const void* ptr) {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeHost;
+}
+
+// Child plan of the same description with `batch` transforms.
+static int child_plan(afft_plan p, int64_t batch, afft_plan* out) {
+    afft_plan_desc d = p->desc;
+    d.batch = batch;
+    return afft_plan_create_desc(static_cast<afft_context>(p->ctx), &d, out);
+}
+
+static bool pipe_eligible(afft_plan p, int di) {
+    const afft_plan_desc& d = p->desc;
+    if (d.rank != 1 || d.batch < 4) return false;
+    if ((d.istride && d.istride != 1) || (d.ostride && d.ostride != 1)) return false;
+    if (p->in_elems[di] % d.batch || p->out_elems[di] % d.batch) return false;
+    const size_t ib = (size_t)p->in_elems[di] * p->in_esize[di];
+    const size_t ob = (size_t)p->out_elems[di] * p->out_esize[di];
+    return std::max(ib, ob) >= kPipeMinBytes;
+}
+
+static int host_exec_pipelined(afft_plan p, int dir, void* dst, const void* src) {
+    const int di = dir == AFFT_INVERSE ? 1 : 0;
+    const int64_t B = p->desc.batch;
+    const size_t ipt = (size_t)(p->in_elems[di] / B) * p->in_esize[di];    // bytes per transform
+    const size_t opt = (size_t)(p->out_elems[di] / B) * p->out_esize[di];
+    if (p->pipe && p->pipe->dir_index != di) {
+        destroy_host_pipe(p->pipe);
+        p->pipe = nullptr;
+    }
+    if (!p->pipe) {
+        auto* h = new HostPipe();
+        h->dir_index = di;
+        h->chunk = std::max<int64_t>(1, (int64_t)(kPipeChunkBytes / std::max(ipt, opt)));
+        h->chunk = std::min<int64_t>(h->chunk, (B + 3) / 4);
+        int rc = AFFT_OK;
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < 2 && !rc && e == hipSuccess; ++i) {
+            rc = child_plan(p, h->chunk, &h->full[i]);
+            if (rc) break;
+            e = hipHostMalloc(&h->hin[i], h->chunk * ipt, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipHostMalloc(&h->hout[i], h->chunk * opt, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipMalloc(&h->din[i], h->chunk * ipt);
+            if (e == hipSuccess) e = hipMalloc(&h->dout[i], h->chunk * opt);
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->s[i], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev[i], hipEventDisableTiming);
+        }
+        if (!rc && e != hipSuccess) rc = set_error(AFFT_ERR_DEVICE, std::string("host pipeline: ") + hipGetErrorString(e));
+        if (rc) {
+            destroy_host_pipe(h);
+            return rc;
+        }
+        p->pipe = h;
+    }
+    HostPipe& h = *p->pipe;
+    const int64_t tail = B % h.chunk;
+    if (tail && h.tail_batch != tail) {
+        if (h.tail) afft_plan_destroy(h.tail);
+        h.tail = nullptr;
+        int rc = child_plan(p, tail, &h.tail);
+        if (rc) return rc;
+        h.tail_batch = tail;
+    }
+    const bool pin_in = host_pinned(src), pin_out = host_pinned(dst);
+    const int64_t nch = (B + h.chunk - 1) / h.chunk;
+    auto count_of = [&](int64_t k) { return std::min<int64_t>(h.chunk, B - k * h.chunk); };
+    // chunk k's output leaves the pinned slot once its event has fired
+    auto drain = [&](int64_t k) -> int {
+        const int slot = (int)(k & 1);
+        AFFT_HIP(hipEventSynchronize(h.ev[slot]));
+        if (!pin_out) std::memcpy((char*)dst + (size_t)(k * h.chunk) * opt, h.hout[slot], (size_t)count_of(k) * opt);
+        return AFFT_OK;
+    };
+    for (int64_t k = 0; k < nch; ++k) {
+        const int slot = (int)(k & 1);
+        if (k >= 2) {
+            int rc = drain(k - 2);
+            if (rc) return rc;
+        }
+        const int64_t nb = count_of(k);
+        afft_plan cp = nb == h.chunk ? h.full[slot] : h.tail;
+        const char* sp = (const char*)src + (size_t)(k * h.chunk) * ipt;
+        char* dp = (char*)dst + (size_t)(k * h.chunk) * opt;
+        const void* up = sp;
+        if (!pin_in) {
+            std::memcpy(h.hin[slot], sp, (size_t)nb * ipt);
+            up = h.hin[slot];
+        }
+        AFFT_HIP(hipMemcpyAsync(h.din[slot], up, (size_t)nb * ipt, hipMemcpyHostToDevice, h.s[slot]));
+        int rc = exec_plan(*cp, dir, h.dout[slot], h.din[slot], h.s[slot]);
+        if (rc) return rc;
+        AFFT_HIP(hipMemcpyAsync(pin_out ? (void*)dp : h.hout[slot], h.dout[slot], (size_t)nb * opt,
+                                hipMemcpyDeviceToHost, h.s[slot]));
+        AFFT_HIP(hipEventRecord(h.ev[slot], h.s[slot]));
+    }
+    for (int64_t k = std::max<int64_t>(0, nch - 2); k < nch; ++k) {
+        int rc = drain(k);
+        if (rc) return rc;
+    }
+    clear_error();
+    return AFFT_OK;
+}
+
 static int host_exec(afft_plan p, int dir, void* dst, int64_t dst_len, const void* src, int64_t src_len) {
     if (!p) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "nil plan");
     if (!dst || !src) return set_error(AFFT_ERR_NIL_SLICE, "dst or src is nil");
@@ -403,6 +576,7 @@ static int host_exec(afft_plan p, int dir, void* dst, int64_t dst_len, const voi
     }
     std::lock_guard<std::mutex> lock(p->mu);
     AFFT_HIP(hipSetDevice(p->ctx->device));
+    if (pipe_eligible(p, di)) return host_exec_pipelined(p, dir, dst, src);
     size_t ib = (size_t)p->in_elems[di] * p->in_esize[di];
     size_t ob = (size_t)p->out_elems[di] * p->out_esize[di];
     if (p->stage_in_bytes < ib) {

@@ -89,7 +89,10 @@ void plan_features(const afft_plan_desc& d, const Candidate& c, float* f) {
     for (int i = 0; i < 4; ++i)
         while (m % q[i] == 0) { m /= q[i]; ++e[i]; }
     const bool smooth = m == 1;
-    const int64_t M = smooth ? n : [&] { int64_t v = 1; while (v < 2 * n - 1) v <<= 1; return v; }();
+    // executed FFT length: the Bluestein padding for a Bluestein candidate
+    // (also for smooth n), else n
+    const bool blue = c.strategy == AFFT_STRATEGY_BLUESTEIN || (c.strategy == AFFT_STRATEGY_AUTO && !smooth);
+    const int64_t M = blue ? [&] { int64_t v = 1; while (v < 2 * n - 1) v <<= 1; return v; }() : n;
     const int64_t stride = d.istride > 1 ? d.istride : 1;
     int64_t n1 = c.split, n2 = c.split ? n / c.split : 0;
     if (c.strategy == AFFT_STRATEGY_SIXSTEP && !n1) { n1 = 1LL << (ilog2i(n) / 2); n2 = n / n1; }
@@ -97,6 +100,18 @@ void plan_features(const afft_plan_desc& d, const Candidate& c, float* f) {
     int launches = 1;
     if (c.strategy == AFFT_STRATEGY_SIXSTEP) launches = 2;
     if (c.strategy == AFFT_STRATEGY_BLUESTEIN) launches = M <= max_single_pass(prec) ? 1 : 4;
+    // radix passes of a length and how many of them are odd (3, 5, 7)
+    auto passes = [](int64_t len, int* odd) {
+        *odd = 0;
+        if (len < 2) return 0;
+        std::vector<int> r = smooth_schedule(len);
+        for (int x : r) *odd += (x & 1);
+        return (int)r.size();
+    };
+    int odd_n = 0, odd1 = 0, odd2 = 0;
+    const int pn = passes(M, &odd_n);
+    const int p1 = passes(n1, &odd1), p2 = passes(n2, &odd2);
+    const bool split = c.strategy == AFFT_STRATEGY_SIXSTEP;
     f[0] = (float)ln;
     f[1] = (float)lb;
     f[2] = (float)(ln + lb);
@@ -104,7 +119,7 @@ void plan_features(const afft_plan_desc& d, const Candidate& c, float* f) {
     f[4] = is_pow2(n) ? 1.f : 0.f;
     for (int i = 0; i < 4; ++i) f[5 + i] = (float)e[i];
     f[9] = smooth ? 1.f : 0.f;
-    f[10] = smooth ? 0.f : 1.f;
+    f[10] = (float)(split ? p2 : 0);
     f[11] = (float)std::log2((double)M);
     f[12] = stride > 1 ? 1.f : 0.f;
     f[13] = (float)std::log2((double)stride);
@@ -119,12 +134,12 @@ void plan_features(const afft_plan_desc& d, const Candidate& c, float* f) {
     f[22] = n <= gen_cap(prec) ? 1.f : 0.f;
     f[23] = (float)launches;
     f[24] = (float)std::log2(es * 2.0 * n * b * launches);
-    f[25] = (float)std::max(0.0, ln + lb - 8.0);
+    f[25] = (split && is_pow2(n1)) ? 1.f : 0.f;
     f[26] = n1 ? (float)std::fabs(std::log2((double)n1) - std::log2((double)n2)) : 0.f;
-    f[27] = (es * 2.0 * n * b > 64.0 * 1024 * 1024) ? 1.f : 0.f;
-    f[28] = (float)(ln * ln / 100.0);
+    f[27] = (split && is_pow2(n2)) ? 1.f : 0.f;
+    f[28] = (float)(split ? p1 : pn);
     f[29] = (float)std::log2(std::max(2.0, (double)M * std::log2((double)std::max<int64_t>(M, 2)) * b));
-    f[30] = (float)std::log2(1.0 + (double)b * n / 65536.0);
+    f[30] = (float)(split ? odd1 + odd2 : odd_n);
     f[31] = 1.f;
 }
 

@@ -63,13 +63,7 @@ def traffic_bytes(n, batch):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
-def mfma_leg(torch, A, ctx, stream, x, steps, n, batch):
-    """The same headline workload through the MFMA form (FP16 hi/lo radix-16
-    products, FP32 accumulation): time and accuracy beside the plain path."""
-    if n not in (256, 4096):
-        return None
-    plan = A.Plan.from_desc(ctx, (n,), precision=A.COMPLEX64, batch=batch, flags=A.FLAG_MFMA)
-    y = torch.empty_like(x)
+def _time_plan(torch, plan, A, stream, y, x, steps):
     sh = stream.cuda_stream
     plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)
     torch.cuda.synchronize()
@@ -79,13 +73,151 @@ def mfma_leg(torch, A, ctx, stream, x, steps, n, batch):
         plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)
     e1.record(stream)
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
-    rows = np.linspace(0, batch - 1, 16).astype(np.int64)
+    return e0.elapsed_time(e1) / steps
+
+
+def _accuracy(x, y, rows):
     ref = np.fft.fft(x[rows].cpu().numpy().astype(np.complex128), axis=1)
-    rel = float(np.linalg.norm(y[rows].cpu().numpy() - ref) / np.linalg.norm(ref))
+    got = y[rows].cpu().numpy().astype(np.complex128)
+    rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    mx = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    return rel, mx
+
+
+def mfma_leg(torch, A, ctx, stream, x, steps, n, batch):
+    """The same headline workload through the MFMA form (FP16 hi/lo radix-16
+    products, FP32 accumulation): time and accuracy beside the plain path."""
+    plan = A.Plan.from_desc(ctx, (n,), precision=A.COMPLEX64, batch=batch, flags=A.FLAG_MFMA)
+    y = torch.empty_like(x)
+    ms = _time_plan(torch, plan, A, stream, y, x, steps)
+    rel, mx = _accuracy(x, y, np.linspace(0, batch - 1, 16).astype(np.int64))
+    algo = plan.describe()
     plan.close()
-    return {"algorithm": f"hip_mfma{n}r", "ms_per_step": round(ms, 5),
-            "gflops": round(5.0 * n * np.log2(n) * batch / (ms * 1e-3) / 1e9, 1), "rel_l2_vs_fp64": rel}
+    return {"algorithm": algo, "ms_per_step": round(ms, 5),
+            "gflops": round(5.0 * n * np.log2(n) * batch / (ms * 1e-3) / 1e9, 1), "rel_l2_vs_fp64": rel,
+            "max_err_vs_fp64": mx}
+
+
+def sweep_leg(torch, A, ctx, stream, steps, x4096=None):
+    """Precision workload at every size: complex64 N in {256, 4096, 65536,
+    2^20}, batch 2^26/N (512 MiB per buffer), plain FP32 and MFMA forms, each
+    timed and checked against numpy's FP64 FFT on sampled rows."""
+    res = []
+    for n in (256, 4096, 65536, 1 << 20):
+        batch = (1 << 26) // n
+        x = x4096 if (n == 4096 and x4096 is not None) else gen_input_device(n, batch, 2024 + n, torch)
+        y = torch.empty_like(x)
+        rows = np.linspace(0, batch - 1, 8).astype(np.int64)
+        for form, flags in (("plain", 0), ("mfma", A.FLAG_MFMA)):
+            plan = A.Plan.from_desc(ctx, (n,), precision=A.COMPLEX64, batch=batch, flags=flags)
+            ms = _time_plan(torch, plan, A, stream, y, x, steps)
+            rel, mx = _accuracy(x, y, rows)
+            gbs = 16.0 * n * batch / (ms * 1e-3) / 1e9
+            res.append({"n": n, "batch": batch, "form": form, "algorithm": plan.describe(), "ms": round(ms, 5),
+                        "gbs_algorithmic": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                        "gflops": round(5.0 * n * np.log2(n) * batch / (ms * 1e-3) / 1e9, 1),
+                        "rel_l2_vs_fp64": rel, "max_err_vs_fp64": mx, "rel_l2_budget": 2e-6 * np.log2(n)})
+            plan.close()
+        del x, y
+        torch.cuda.empty_cache()
+    return res
+
+
+def nd_leg(torch, A, ctx, stream, steps):
+    """N-D, real and FP64 workloads: float32 R2C 512^3, complex128 C2C
+    8192^2 and complex128 Bluestein N=1,000,003 x 8, each forward, device
+    resident; algorithmic bytes as sizeof(in) + sizeof(out)."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(11)
+    res = []
+
+    def run(name, plan, x, y, alg_bytes, flops, check):
+        ms = _time_plan(torch, plan, A, stream, y, x, steps)
+        gbs = alg_bytes / (ms * 1e-3) / 1e9
+        entry = {"workload": name, "algorithm": plan.describe(), "ms": round(ms, 4),
+                 "gbs_algorithmic": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                 "gflops": round(flops / (ms * 1e-3) / 1e9, 1)}
+        entry.update(check())
+        res.append(entry)
+
+    d = 512
+    xr = torch.rand((d, d, d), device=dev, generator=g) * 2 - 1
+    X = torch.empty((d, d, d // 2 + 1), dtype=torch.complex64, device=dev)
+    plan = A.Plan.from_desc(ctx, (d, d, d), kind=A.R2C)
+
+    def chk_r2c():
+        # Parseval over the half spectrum (interior bins twice)
+        Xd = X.to(torch.complex128)
+        e = (Xd.abs() ** 2).sum() * 2 - (Xd[..., 0].abs() ** 2).sum() - (Xd[..., -1].abs() ** 2).sum()
+        t = (xr.to(torch.float64) ** 2).sum() * d ** 3
+        return {"parseval_rel_dev": abs(float(e / t) - 1.0)}
+
+    run("float32 R2C 3-D 512x512x512", plan, xr, X, 4 * d ** 3 + 8 * d * d * (d // 2 + 1),
+        2.5 * d ** 3 * np.log2(float(d ** 3)), chk_r2c)
+    plan.close()
+    del xr, X
+    n = 8192
+    x = torch.complex(torch.randn((n, n), device=dev, dtype=torch.float64, generator=g),
+                      torch.randn((n, n), device=dev, dtype=torch.float64, generator=g))
+    y = torch.empty_like(x)
+    plan = A.Plan.from_desc(ctx, (n, n), precision=A.COMPLEX128)
+
+    def chk_2d():
+        rel = float((y.abs() ** 2).sum() / ((x.abs() ** 2).sum() * n * n)) - 1.0
+        k1 = 77
+        colmix = np.exp(-2j * np.pi * k1 * np.arange(n) / n) @ x.cpu().numpy()
+        want = np.fft.fft(colmix)
+        got = y[k1].cpu().numpy()
+        return {"parseval_rel_dev": abs(rel), "sampled_row_rel_l2": float(np.linalg.norm(got - want) / np.linalg.norm(want))}
+
+    run("complex128 C2C 2-D 8192x8192", plan, x, y, 2 * 16 * n * n, 5.0 * n * n * np.log2(float(n * n)), chk_2d)
+    plan.close()
+    del x, y
+    n, b = 1_000_003, 8
+    x = torch.complex(torch.randn((b, n), device=dev, dtype=torch.float64, generator=g),
+                      torch.randn((b, n), device=dev, dtype=torch.float64, generator=g))
+    y = torch.empty_like(x)
+    plan = A.Plan.from_desc(ctx, (n,), precision=A.COMPLEX128, batch=b)
+
+    def chk_blue():
+        ref = np.fft.fft(x[:2].cpu().numpy(), axis=1)
+        got = y[:2].cpu().numpy()
+        return {"rel_l2_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)), "rel_l2_budget": 1e-13}
+
+    run("complex128 Bluestein N=1000003 batch 8", plan, x, y, 2 * 16 * n * b, 5.0 * n * np.log2(n) * b, chk_blue)
+    plan.close()
+    del x, y
+    torch.cuda.empty_cache()
+    return res
+
+
+def host_leg(torch, A, ctx, n, batch, reps=3):
+    """End to end from host memory: H2D, transform, D2H through the host-slice
+    entry point (afft_plan_forward), from pageable and from pinned buffers."""
+    import ctypes
+    plan = A.Plan.from_desc(ctx, (n,), precision=A.COMPLEX64, batch=batch)
+    total = n * batch
+    out = {"workload": f"complex64 N={n} batch={batch} host in, host out", "bytes_each_way": 8 * total}
+    for kind in ("pageable", "pinned"):
+        pin = kind == "pinned"
+        xs = torch.empty(total, dtype=torch.complex64, pin_memory=pin)
+        xs.real.uniform_(-1, 1)
+        xs.imag.uniform_(-1, 1)
+        ys = torch.empty(total, dtype=torch.complex64, pin_memory=pin)
+        fn = A.lib().afft_plan_forward
+        args = (plan.h, ctypes.c_void_p(ys.data_ptr()), total, ctypes.c_void_p(xs.data_ptr()), total)
+        A.check(fn(*args))
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            A.check(fn(*args))
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        out[kind] = {"ms": round(t * 1e3, 3), "gflops": round(5.0 * n * np.log2(n) * batch / t / 1e9, 1),
+                     "gbs_each_way": round(8 * total / t / 1e9, 2)}
+        del xs, ys
+    plan.close()
+    return out
 
 
 def ragged_leg(torch, A, ctx, stream, steps):
@@ -164,6 +296,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-ragged", action="store_true", help="skip the mixed-size leg")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the precision sweep over N")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-memory end-to-end leg")
+    ap.add_argument("--no-nd", action="store_true", help="skip the N-D / real / FP64 leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -265,10 +400,18 @@ def main():
     }
     plan.close()
     out["mfma"] = mfma_leg(torch, A, ctx, stream, x, args.steps, n, batch)
-    if out["mfma"]:
-        out["mfma"]["speed_vs_plain"] = round(ms / out["mfma"]["ms_per_step"], 3)
+    out["mfma"]["speed_vs_plain"] = round(ms / out["mfma"]["ms_per_step"], 3)
     if not args.no_ragged:
         out["ragged"] = ragged_leg(torch, A, ctx, stream, max(3, args.steps // 5))
+    if not args.no_sweep:
+        out["precision_sweep"] = sweep_leg(torch, A, ctx, stream, max(5, args.steps // 5),
+                                           x if (n == 4096 and batch == (1 << 26) // 4096) else None)
+    del y
+    torch.cuda.empty_cache()
+    if not args.no_nd:
+        out["nd_real_fp64"] = nd_leg(torch, A, ctx, stream, max(3, args.steps // 10))
+    if not args.no_host:
+        out["host_end_to_end"] = host_leg(torch, A, ctx, n, batch)
     ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)

@@ -142,6 +142,12 @@ int afft_buffer_download(afft_buffer buf, void* host, int64_t host_len);
 void* afft_buffer_device_ptr(afft_buffer buf);
 int afft_buffer_destroy(afft_buffer buf);                       /* Buffer.Close */
 
+/* Page-locked host memory (no counterpart in the reference: its gpu package
+ * only takes Go slices).  Host slices in such memory are transferred by DMA
+ * directly, without staging, by afft_plan_forward / afft_plan_inverse. */
+int afft_host_alloc(afft_context ctx, int64_t bytes, void** out);
+int afft_host_free(void* ptr);
+
 /* ---- Stream (gpu/backend.go:38-41; Context.NewStream :20) ---------------- */
 int afft_stream_create(afft_context ctx, afft_stream* out);
 int afft_stream_synchronize(afft_stream s);                     /* Stream.Synchronize */

@@ -1,5 +1,6 @@
 """The full-size workloads: the precision sweep (complex64 N in {256, 4096,
-65536, 2^20}, batch 2^26/N, 8 seeded tones + noise), float32 R2C 512^3,
+65536, 2^20}, batch 2^26/N, 8 seeded tones + noise, plain FP32 and MFMA
+forms), float32 R2C 512^3,
 complex128 C2C 8192^2 and complex128 Bluestein N=1,000,003 x 8.
 
 At full size the oracle checks sampled rows (C2C sweep, Bluestein in full);
@@ -28,8 +29,9 @@ def _exec(plan, direction, out, x, stream):
     stream.synchronize()
 
 
+@pytest.mark.parametrize("form", ["plain", "mfma"])
 @pytest.mark.parametrize("n", [256, 4096, 65536, 1 << 20])
-def test_precision_sweep_full_batch(ctx, n):
+def test_precision_sweep_full_batch(ctx, n, form):
     batch = (1 << 26) // n
     rows = 4 if n >= 65536 else 16
     # full-size device input: the seeded tones for sampled rows are checked
@@ -42,14 +44,15 @@ def test_precision_sweep_full_batch(ctx, n):
     xt[torch.from_numpy(pick).cuda()] = _dev(x)
     yt = torch.empty_like(xt)
     s = torch.cuda.Stream()
-    with A.Plan.from_desc(ctx, (n,), batch=batch) as p:
+    flags = A.FLAG_MFMA if form == "mfma" else 0
+    with A.Plan.from_desc(ctx, (n,), batch=batch, flags=flags) as p:
         _exec(p, A.FORWARD, yt, xt, s)
         algo = p.describe()
     y = yt[torch.from_numpy(pick).cuda()].cpu().numpy()
     ref = O.fft_batch(x.astype(np.complex128))
     err = rel_l2(y, ref)
     maxerr = float(np.max(np.abs(y - ref)) / np.max(np.abs(ref)))
-    print(f"n={n} batch={batch} algo={algo} relL2={err:.3e} max={maxerr:.3e} budget={c64_tol(n):.1e}")
+    print(f"{form} n={n} batch={batch} algo={algo} relL2={err:.3e} max={maxerr:.3e} budget={c64_tol(n):.1e}")
     assert err <= c64_tol(n)
 
 

@@ -21,12 +21,13 @@ def _run(ctx, x, flags, inverse=False):
         return y, p.describe()
 
 
-@pytest.mark.parametrize("n", [256, 4096])
+@pytest.mark.parametrize("n", [256, 1024, 4096, 65536, 1 << 18, 1 << 20])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_mfma_matches_oracle(ctx, n, inverse):
-    x = synth.uniform_complex(n, (64, n), np.complex64)
+    batch = max(2, min(64, (1 << 21) // n))
+    x = synth.uniform_complex(n, (batch, n), np.complex64)
     y, algo = _run(ctx, x, A.FLAG_MFMA, inverse)
-    assert algo.startswith("hip_mfma"), algo
+    assert "mfma" in algo, algo
     ref = O.fft_batch(x.astype(np.complex128), inverse=inverse)
     err = rel_l2(y, ref)
     plain, _ = _run(ctx, x, 0, inverse)
@@ -34,20 +35,33 @@ def test_mfma_matches_oracle(ctx, n, inverse):
     assert err <= c64_tol(n)
 
 
-@pytest.mark.parametrize("n", [256, 4096])
+@pytest.mark.parametrize("n", [256, 1024, 4096, 65536, 1 << 20])
 def test_mfma_on_high_dynamic_range_tones(ctx, n):
-    x, bins, amps = synth.sinusoids(77, n, 32)
+    rows = 32 if n <= 4096 else 4
+    x, bins, amps = synth.sinusoids(77, n, rows)
     y, _ = _run(ctx, x, A.FLAG_MFMA)
     ref = O.fft_batch(x.astype(np.complex128))
     assert rel_l2(y, ref) <= c64_tol(n)
     # the strongest tone of every row lands in its bin
-    for b in range(32):
+    for b in range(rows):
         j = int(np.argmax(amps[b]))
         assert abs(y[b, bins[b, j]]) >= 0.5 * n * amps[b, j]
 
 
+def test_mfma_strided_four_step_slices(ctx):
+    # a batch of 65536-point rows at a row distance larger than n
+    n, batch, dist = 65536, 3, 65536 + 512
+    x = synth.uniform_complex(9, (batch, dist), np.complex64)
+    with A.Plan.from_desc(ctx, (n,), batch=batch, idist=dist, odist=dist, flags=A.FLAG_MFMA) as p:
+        y = np.zeros_like(x)
+        p.forward(y, x)
+    ref = O.fft_batch(x[:, :n].astype(np.complex128))
+    assert rel_l2(y[:, :n], ref) <= c64_tol(n)
+
+
 def test_mfma_unsupported_shapes_are_reported(ctx):
-    with pytest.raises(A.ErrNotImplemented):
-        A.Plan.from_desc(ctx, (1024,), flags=A.FLAG_MFMA)
+    for n in (2048, 8192, 1000, 1 << 22):
+        with pytest.raises(A.ErrNotImplemented):
+            A.Plan.from_desc(ctx, (n,), flags=A.FLAG_MFMA)
     with pytest.raises(A.ErrNotImplemented):
         A.Plan.from_desc(ctx, (4096,), precision=A.COMPLEX128, flags=A.FLAG_MFMA)
