@@ -251,9 +251,46 @@ class Plan:
     def exec_buffers(self, direction, dst, src, stream=None):
         check(lib().afft_plan_exec_buffers(self.h, direction, dst.h, src.h, stream.h if stream else None))
 
+    def clone(self):
+        """Plan.Clone / NewExecutor: same description, own stream and workspace."""
+        h = ctypes.c_void_p()
+        check(lib().afft_plan_clone(self.h, ctypes.byref(h)))
+        return Plan(h, self.desc)
+
     def close(self):
         if self.h:
             check(lib().afft_plan_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class FastPlan:
+    """FastPlan (plan_fast.go): one pre-resolved launch per direction, device
+    pointers, no validation and no lock."""
+
+    def __init__(self, ctx, n, batch=1, precision=COMPLEX64, stream=None):
+        h = ctypes.c_void_p()
+        check(lib().afft_fast_plan_create(ctx.h, n, batch, precision, ctypes.c_void_p(stream) if stream else None,
+                                          ctypes.byref(h)))
+        self.h = h
+
+    def __len__(self):
+        return lib().afft_fast_plan_len(self.h)
+
+    def forward(self, dst_ptr, src_ptr):
+        lib().afft_fast_forward(self.h, ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr))
+
+    def inverse(self, dst_ptr, src_ptr):
+        lib().afft_fast_inverse(self.h, ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr))
+
+    def close(self):
+        if self.h:
+            check(lib().afft_fast_plan_destroy(self.h))
             self.h = None
 
     def __enter__(self):

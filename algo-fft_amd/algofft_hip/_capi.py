@@ -67,6 +67,12 @@ SIGNATURES = [
     ("afft_plan_describe", c_int, [c_void_p, ctypes.c_char_p, c_int]),
     ("afft_plan_workspace_bytes", c_i64, [c_void_p]),
     ("afft_plan_destroy", c_int, [c_void_p]),
+    ("afft_plan_clone", c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
+    ("afft_fast_plan_create", c_int, [c_void_p, c_i64, c_i64, c_int, c_void_p, ctypes.POINTER(c_void_p)]),
+    ("afft_fast_plan_len", c_i64, [c_void_p]),
+    ("afft_fast_forward", None, [c_void_p, c_void_p, c_void_p]),
+    ("afft_fast_inverse", None, [c_void_p, c_void_p, c_void_p]),
+    ("afft_fast_plan_destroy", c_int, [c_void_p]),
     ("afft_ragged_create", c_int, [c_void_p, c_int, ctypes.POINTER(c_i64), c_i64, c_int, ctypes.POINTER(c_void_p)]),
     ("afft_ragged_total_len", c_i64, [c_void_p]),
     ("afft_ragged_offsets", c_int, [c_void_p, ctypes.POINTER(c_i64), c_i64]),

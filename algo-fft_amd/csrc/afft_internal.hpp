@@ -83,6 +83,7 @@ struct Step {
     int64_t in_off = 0, out_off = 0;
     std::vector<unsigned char> arg;   // kernel argument block (a POD struct)
     int in_field = 0, out_field = 0;  // byte offsets of the in/out pointers in arg
+    int nt_field = -1;                // byte offset of KArgs::nt in arg, if any
     std::string name;
 };
 

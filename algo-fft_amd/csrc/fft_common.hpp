@@ -20,6 +20,19 @@ struct __attribute__((aligned(2 * sizeof(T)))) cx {
 
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+
+// Output store; nt marks it non-temporal (no allocation in the last-level
+// cache), for outputs too large to be re-read from there.
+template <typename T>
+__device__ __forceinline__ void st_out(cx<T>* p, cx<T> v, int nt) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    if (nt) {
+        const v2 val = {v.x, v.y};
+        __builtin_nontemporal_store(val, (v2*)p);
+    } else {
+        *p = v;
+    }
+}
 template <typename T>
 __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
 template <typename T>

@@ -41,6 +41,7 @@ struct KArgs {
     int fconj;               // use conj(fvec)
     const void* fvec;
     long long flen;
+    int nt;                  // non-temporal stores of the output
 };
 
 enum { FUSE_NONE = 0, FUSE_CHIRP_IN = 1, FUSE_MUL_OUT = 2, FUSE_CROP_OUT = 3 };
@@ -140,14 +141,14 @@ __global__ void __launch_bounds__((KCfg<T, S, E, TPB, VAR>::THREADS)) k_fft(KArg
         if (act) {
             cx<T>* dst = out + cg * a.out_cd;
 #pragma unroll
-            for (int e = 0; e < E; ++e) dst[t + TT * e] = v[e];
+            for (int e = 0; e < E; ++e) st_out(dst + t + TT * e, v[e], a.nt);
         }
     } else if constexpr (VAR == VAR_COL || VAR == VAR_4A) {
         if (act) {
             cx<T>* dst = out + cg;
             const long long nd = a.out_nd;
 #pragma unroll
-            for (int e = 0; e < E; ++e) dst[(long long)(t + TT * e) * nd] = v[e];
+            for (int e = 0; e < E; ++e) st_out(dst + (long long)(t + TT * e) * nd, v[e], a.nt);
         }
     } else {  // VAR_4B: re-map through LDS so lanes run along c on the store
         LdsRow<N, C::PADSH> li(c);
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__((KCfg<T, S, E, TPB, VAR>::THREADS)) k_fft(KArg
             const long long nd = a.out_nd;
             if (a.fuse == FUSE_NONE) {
 #pragma unroll
-                for (int e = 0; e < E; ++e) out[cg2 + (long long)(t2 + TT * e) * nd] = lds[li2(t2 + TT * e)];
+                for (int e = 0; e < E; ++e) st_out(out + cg2 + (long long)(t2 + TT * e) * nd, lds[li2(t2 + TT * e)], a.nt);
             } else {
 #pragma unroll
                 for (int e = 0; e < E; ++e) {

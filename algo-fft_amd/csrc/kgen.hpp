@@ -124,9 +124,9 @@ __global__ void __launch_bounds__(GEN_THREADS) k_gen(GArgs g) {
             }
             if (a.scale != 1.0) y = scl(y, (T)a.scale);
             if constexpr (VAR == VAR_ROW) {
-                if (act) out[cg * a.out_cd + k] = y;
+                if (act) st_out(out + cg * a.out_cd + k, y, a.nt);
             } else if constexpr (VAR == VAR_COL || VAR == VAR_4A) {
-                if (act) out[cg + (long long)k * a.out_nd] = y;
+                if (act) st_out(out + cg + (long long)k * a.out_nd, y, a.nt);
             } else {
                 sm[lidx(wb, k)] = y;   // re-mapped below
             }
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(GEN_THREADS) k_gen(GArgs g) {
         const int c2 = tid % G, t2 = tid / G;
         const long long cg2 = (long long)blockIdx.x * G + c2;
         if (cg2 < a.count) {
-            for (int k = t2; k < N; k += TPT) out[cg2 + (long long)k * a.out_nd] = sm[fb * G * N + c2 * N + k];
+            for (int k = t2; k < N; k += TPT) st_out(out + cg2 + (long long)k * a.out_nd, sm[fb * G * N + c2 * N + k], a.nt);
         }
     }
 }

@@ -154,7 +154,7 @@ __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, const ha
             for (int i = 0; i < 4; ++i) {
                 const int idx = base + ko[i] * NS;
                 if constexpr (GOUT) {
-                    if (cg < a.count) out[cg * a.out_cd + idx] = scl(y[it][i], (float)a.scale);
+                    if (cg < a.count) st_out(out + cg * a.out_cd + idx, scl(y[it][i], (float)a.scale), a.nt);
                 } else {
                     lds[c * C::PITCH + mf_pad(idx)] = y[it][i];
                 }
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__((MfCfg<N, G>::T)) k_mfma(KArgs a) {
             if (cg < a.count) {
                 cx<float> v = lds[c * C::PITCH + mf_pad(k)];
                 if constexpr (VAR == VAR_4A) v = twmul<DIR>(v, tw4<float>(a, (long long)k * (cg + a.tw4_c0)));
-                out[cg * a.out_cd + (long long)k * a.out_nd] = scl(v, s);
+                st_out(out + cg * a.out_cd + (long long)k * a.out_nd, scl(v, s), a.nt);
             }
         }
     }

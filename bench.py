@@ -191,6 +191,32 @@ def nd_leg(torch, A, ctx, stream, steps):
     return res
 
 
+def latency_leg(torch, A, ctx, stream, calls=2000):
+    """Host time per call for a small transform (complex64 N=256, one row):
+    the checked plan entry point against the fast plan (plan_fast.go), both
+    through ctypes, with the device synchronised once at the end."""
+    n = 256
+    x = torch.zeros(n, dtype=torch.complex64, device="cuda:0")
+    y = torch.empty_like(x)
+    sh = stream.cuda_stream
+    res = {"workload": "complex64 N=256 batch 1, device resident", "calls": calls}
+    plan = A.Plan.from_desc(ctx, (n,))
+    fp = A.FastPlan(ctx, n, 1, A.COMPLEX64, sh)
+    for name, fn in (("checked_us", lambda: plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)),
+                     ("fast_us", lambda: fp.forward(y.data_ptr(), x.data_ptr()))):
+        for _ in range(50):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = round((time.perf_counter() - t0) / calls * 1e6, 3)
+    fp.close()
+    plan.close()
+    return res
+
+
 def host_leg(torch, A, ctx, n, batch, reps=3):
     """End to end from host memory: H2D, transform, D2H through the host-slice
     entry point (afft_plan_forward), from pageable and from pinned buffers."""
@@ -410,6 +436,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_nd:
         out["nd_real_fp64"] = nd_leg(torch, A, ctx, stream, max(3, args.steps // 10))
+    out["latency_small_n"] = latency_leg(torch, A, ctx, stream)
     if not args.no_host:
         out["host_end_to_end"] = host_leg(torch, A, ctx, n, batch)
     ctx.close()

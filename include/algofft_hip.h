@@ -188,6 +188,28 @@ int afft_plan_describe(afft_plan p, char* buf, int cap);
 int64_t afft_plan_workspace_bytes(afft_plan p);
 int afft_plan_destroy(afft_plan p);           /* PlanImpl.Close */
 
+/* Plan.Clone / Plan.NewExecutor (plan.go:1300, executor.go:6-43): a plan of
+ * the same description with its own stream and workspace, so host threads
+ * that each use their own clone never contend for one plan. */
+int afft_plan_clone(afft_plan p, afft_plan* out);
+
+/* ---- FastPlan (plan_fast.go:21-133) ---------------------------------------
+ * Power-of-two plans that run as one kernel launch, with the launch resolved
+ * at creation.  NewFastPlan's errors: ErrInvalidLength unless n >= 1 is a
+ * power of two, ErrNotImplemented when no single-launch kernel covers n
+ * (n > 16384 complex64 / 8192 complex128).  batch transforms of n contiguous
+ * points.  Forward / Inverse take device pointers, validate nothing and take
+ * no lock: the caller guarantees valid buffers of n*batch elements, the
+ * plan's device current, and one host thread per fast plan.  In place is
+ * allowed.  Launches go to `stream` (NULL: the fast plan's own). */
+typedef struct afft_fast_plan_s* afft_fast_plan;
+int afft_fast_plan_create(afft_context ctx, int64_t n, int64_t batch, int precision, void* stream,
+                          afft_fast_plan* out);
+int64_t afft_fast_plan_len(afft_fast_plan fp);
+void afft_fast_forward(afft_fast_plan fp, void* dst, const void* src);
+void afft_fast_inverse(afft_fast_plan fp, void* dst, const void* src);
+int afft_fast_plan_destroy(afft_fast_plan fp);
+
 /* ---- ragged batches ------------------------------------------------------
  * No single counterpart in the reference: a caller there loops Plan.Forward
  * (plan.go:332) over plans of different sizes, as ForwardBatch loops one size

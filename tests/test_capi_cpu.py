@@ -65,3 +65,13 @@ def test_no_device_means_unavailable():
         pytest.skip("a GPU is visible")
     with pytest.raises(A.ErrBackendUnavailable):
         A.Context(0)
+
+
+def test_fast_plan_length_checks_come_first():
+    # NewFastPlan (plan_fast.go:47-49): ErrInvalidLength before anything else
+    import ctypes
+    h = ctypes.c_void_p()
+    for n in (0, -4, 1000, 3):
+        assert A.lib().afft_fast_plan_create(None, n, 1, A.COMPLEX64, None, ctypes.byref(h)) == 4
+    assert A.lib().afft_fast_plan_create(None, 1024, 0, A.COMPLEX64, None, ctypes.byref(h)) == 4
+    assert A.lib().afft_fast_plan_create(None, 1024, 1, A.COMPLEX64, None, ctypes.byref(h)) == 10
