@@ -33,6 +33,7 @@ struct KArgs {
     int tw4_shift;
     long long tw4_L;
     long long tw4_c0;        // column index offset of this launch
+    int tw4_by_y;            // 1: the four-step column index is blockIdx.y, not c
     // Bluestein fusion on the four-step passes (runtime, uniform per launch):
     //   FUSE_CHIRP_IN (VAR_4A): x[n] * chirp[n] for n < flen, zero beyond
     //   FUSE_MUL_OUT  (VAR_4B): y[k] * fvec[k]
@@ -122,7 +123,7 @@ __global__ void __launch_bounds__((KCfg<T, S, E, TPB, VAR>::THREADS)) k_fft(KArg
     }
 
     if constexpr (VAR == VAR_4A) {
-        const long long col = cg + a.tw4_c0;
+        const long long col = (a.tw4_by_y ? (long long)blockIdx.y : cg) + a.tw4_c0;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             // k < N and col < L/N, so k*col < L needs no reduction

@@ -127,6 +127,27 @@ def test_single_strided(ctx):
     assert rel_l2(out[idx], ref[idx]) <= C128_TOL
 
 
+@pytest.mark.parametrize("dims,dtype", [((4096, 64), np.complex64), ((16384, 16), np.complex64),
+                                        ((2048, 32), np.complex128), ((3, 4096, 20), np.complex64),
+                                        ((8192, 17), np.complex128)])
+def test_long_column_axes(ctx, dims, dtype):
+    # axes too long for 8 adjacent columns per block run as a column
+    # four-step (two coalesced passes); checked against the PlanND oracle
+    prec = A.COMPLEX64 if dtype == np.complex64 else A.COMPLEX128
+    x = synth.uniform_complex(7, dims, dtype)
+    with A.Plan.from_desc(ctx, dims, precision=prec) as p:
+        algo = p.describe()
+        out = np.empty_like(x)
+        p.forward(out, x)
+        back = np.empty_like(x)
+        p.inverse(back, out)
+    n = int(np.prod(dims))
+    tol = c64_tol(n) if dtype == np.complex64 else C128_TOL
+    assert "colfourstep" in algo, algo
+    assert rel_l2(out, O.fftn(x.astype(np.complex128))) <= tol
+    assert rel_l2(back, x) <= 2 * tol
+
+
 @pytest.mark.parametrize("dims", [(64, 128), (512, 512), (16, 32, 64), (8, 4, 16, 32)])
 def test_nd_c2c(ctx, dims):
     x = synth.uniform_complex(2, dims, np.complex64)
