@@ -185,36 +185,32 @@ def nd_leg(torch, A, ctx, stream, steps):
         return {"rel_l2_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)), "rel_l2_budget": 1e-13}
 
     run("complex128 Bluestein N=1000003 batch 8", plan, x, y, 2 * 16 * n * b, 5.0 * n * np.log2(n) * b, chk_blue)
+    # executed traffic of the chained form per transform, in points of the
+    # padded M = 2^21: pass A1 writes M; B1 reads M and the M-point filter
+    # and writes M; A2 reads and writes M; B2 reads M (8M), plus the N-point
+    # input, chirp, post-chirp and output (4N)
+    m = 1 << 21
+    res[-1]["executed_bytes"] = (8 * m + 4 * n) * 16 * b
+    res[-1]["gbs_executed"] = round(res[-1]["executed_bytes"] / (res[-1]["ms"] * 1e-3) / 1e9, 1)
     plan.close()
     del x, y
     torch.cuda.empty_cache()
     return res
 
 
-def latency_leg(torch, A, ctx, stream, calls=2000):
-    """Host time per call for a small transform (complex64 N=256, one row):
-    the checked plan entry point against the fast plan (plan_fast.go), both
-    through ctypes, with the device synchronised once at the end."""
-    n = 256
-    x = torch.zeros(n, dtype=torch.complex64, device="cuda:0")
-    y = torch.empty_like(x)
-    sh = stream.cuda_stream
-    res = {"workload": "complex64 N=256 batch 1, device resident", "calls": calls}
-    plan = A.Plan.from_desc(ctx, (n,))
-    fp = A.FastPlan(ctx, n, 1, A.COMPLEX64, sh)
-    for name, fn in (("checked_us", lambda: plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)),
-                     ("fast_us", lambda: fp.forward(y.data_ptr(), x.data_ptr()))):
-        for _ in range(50):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(calls):
-            fn()
-        torch.cuda.synchronize()
-        res[name] = round((time.perf_counter() - t0) / calls * 1e6, 3)
-    fp.close()
-    plan.close()
-    return res
+def latency_leg():
+    """Host cost per call of a small device-resident transform (complex64
+    N=256, one row) from C++: the checked entry point against the fast plan
+    (plan_fast.go), enqueue-only and launch-and-wait."""
+    import subprocess
+    exe = os.path.join(ROOT, "algo-fft_amd", "lib", "latency_bench")
+    if not os.path.exists(exe):
+        return {"error": "latency_bench not built"}
+    r = subprocess.run([exe, "256", "20000"], capture_output=True, text=True, timeout=120)
+    try:
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": (r.stdout + r.stderr)[-300:]}
 
 
 def host_leg(torch, A, ctx, n, batch, reps=3):
@@ -325,6 +321,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the precision sweep over N")
     ap.add_argument("--no-host", action="store_true", help="skip the host-memory end-to-end leg")
     ap.add_argument("--no-nd", action="store_true", help="skip the N-D / real / FP64 leg")
+    ap.add_argument("--no-latency", action="store_true", help="skip the small-transform latency probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -436,7 +433,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_nd:
         out["nd_real_fp64"] = nd_leg(torch, A, ctx, stream, max(3, args.steps // 10))
-    out["latency_small_n"] = latency_leg(torch, A, ctx, stream)
+    if not args.no_latency:
+        out["latency_small_n"] = latency_leg()
     if not args.no_host:
         out["host_end_to_end"] = host_leg(torch, A, ctx, n, batch)
     ctx.close()
