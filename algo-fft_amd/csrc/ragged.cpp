@@ -2,10 +2,11 @@
 // lengths, packed back to back (job i at offset sum of the earlier lengths).
 //
 // Three ways to run the same batch:
-//   afft_ragged_exec        the persistent work-queue kernel, one launch per
-//                           phase (whole jobs and column tasks; row tasks;
-//                           the Bluestein jobs' second column tasks), after
-//                           one counter-reset kernel
+//   afft_ragged_exec        the persistent work-queue kernels, one launch per
+//                           task type (column tasks, row tasks, the
+//                           Bluestein jobs' second column tasks, whole jobs;
+//                           plain and Bluestein jobs apart), after one
+//                           counter-reset kernel
 //   afft_ragged_exec_graph  the same launches captured once in a hipGraph
 //                           and replayed as one graph launch
 //   afft_ragged_exec_per_size  every job through the plan of its own size,
@@ -22,7 +23,7 @@
 
 using namespace afft;
 
-constexpr int RAG_PHASES = 3;
+constexpr int RAG_PHASES = RL_COUNT;
 
 struct afft_ragged_s {
     afft_context ctx = nullptr;
@@ -35,12 +36,12 @@ struct afft_ragged_s {
     RJob* jobs_dev = nullptr;
     RTask* tasks_dev = nullptr;
     int njobs = 0;
-    int ntask[RAG_PHASES] = {0, 0, 0};
-    int task_off[RAG_PHASES] = {0, 0, 0};
+    int ntask[RAG_PHASES] = {};
+    int task_off[RAG_PHASES] = {};
+    int grid[2][RAG_PHASES] = {};       // [inverse][launch]
     int* counter = nullptr;
     void* scratch = nullptr;
     int64_t scratch_pts = 0;
-    int grid = 0;
     hipStream_t stream = nullptr;
     std::map<std::tuple<int, void*, const void*, void*>, hipGraphExec_t> graphs;
     std::map<int64_t, afft_plan> per_size;
@@ -130,6 +131,24 @@ static int pow2_floor(int64_t v) {
     return g;
 }
 
+// The persistent kernel of launch p (RL_*) for direction dir.
+template <typename T>
+static const void* rag_kernel_t(int p, int dir) {
+    switch (p) {
+    case RL_COLS: return dir > 0 ? (const void*)&k_rag_cols<T, 1> : (const void*)&k_rag_cols<T, -1>;
+    case RL_COLS_BLUE: return (const void*)&k_rag_cols<T, -1>;
+    case RL_ROWS: return dir > 0 ? (const void*)&k_rag_rows<T, 1> : (const void*)&k_rag_rows<T, -1>;
+    case RL_ROWS_BLUE: return (const void*)&k_rag_rows_blue<T>;
+    case RL_COLS2: return (const void*)&k_rag_cols2<T>;
+    case RL_WHOLE: return dir > 0 ? (const void*)&k_rag_whole<T, 1> : (const void*)&k_rag_whole<T, -1>;
+    default: return (const void*)&k_rag_whole_blue<T>;
+    }
+}
+
+static const void* rag_kernel(int prec, int p, int dir) {
+    return prec ? rag_kernel_t<double>(p, dir) : rag_kernel_t<float>(p, dir);
+}
+
 static int ragged_build(afft_ragged_s& r) {
     const int es = r.prec ? 16 : 8;
     std::vector<RJob> jobs(r.sizes.size());
@@ -179,8 +198,10 @@ static int ragged_build(afft_ragged_s& r) {
             j.shift = tt->second.shift;
             j.twlo = tt->second.lo;
             j.twhi = tt->second.hi;
-            j.g_col = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_HALF / (m1 + 1), m2}));
-            j.g_row = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_HALF / (m2 + 1), m1}));
+            // at most RAG_PTS points per task: every thread then holds at
+            // most 16/R butterflies of a radix-R pass
+            j.g_col = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m1, m2}));
+            j.g_row = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m2, m1}));
             j.scr_off = r.scratch_pts;
             r.scratch_pts += m;
         }
@@ -201,7 +222,7 @@ static int ragged_build(afft_ragged_s& r) {
         const int job = (int)i;
         if (whole) {
             const double c = (blue ? 2.0 : 1.0) * m * std::log2(std::max<double>(2.0, (double)m));
-            ph[0].push_back({{job, RT_WHOLE, 0, 1}, c});
+            ph[blue ? RL_WHOLE_BLUE : RL_WHOLE].push_back({{job, RT_WHOLE, 0, 1}, c});
             continue;
         }
         // staging adds about one pass of LDS traffic to every task
@@ -209,12 +230,12 @@ static int ragged_build(afft_ragged_s& r) {
         const double crow = (double)m2 * (std::log2((double)m2) + 2.0) * (blue ? 2.0 : 1.0);
         for (int64_t c0 = 0; c0 < m2; c0 += j.g_col) {
             const int cnt = (int)std::min<int64_t>(j.g_col, m2 - c0);
-            ph[0].push_back({{job, RT_COLS, (int)c0, cnt}, ccol * cnt});
-            if (blue) ph[2].push_back({{job, RT_COLS2, (int)c0, cnt}, ccol * cnt});
+            ph[blue ? RL_COLS_BLUE : RL_COLS].push_back({{job, RT_COLS, (int)c0, cnt}, ccol * cnt});
+            if (blue) ph[RL_COLS2].push_back({{job, RT_COLS2, (int)c0, cnt}, ccol * cnt});
         }
         for (int64_t r0 = 0; r0 < m1; r0 += j.g_row) {
             const int cnt = (int)std::min<int64_t>(j.g_row, m1 - r0);
-            ph[1].push_back({{job, RT_ROWS, (int)r0, cnt}, crow * cnt});
+            ph[blue ? RL_ROWS_BLUE : RL_ROWS].push_back({{job, RT_ROWS, (int)r0, cnt}, crow * cnt});
         }
     }
     r.total = off;
@@ -233,10 +254,13 @@ static int ragged_build(afft_ragged_s& r) {
     if (!tasks.empty())
         AFFT_HIP(hipMemcpy(r.tasks_dev, tasks.data(), tasks.size() * sizeof(RTask), hipMemcpyHostToDevice));
     AFFT_HIP(hipMalloc(&r.counter, 64));
-    const void* fn = r.prec ? (const void*)&k_ragged<double> : (const void*)&k_ragged<float>;
-    int per_cu = 0;
-    AFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, RAG_THREADS, 0));
-    r.grid = std::max(1, per_cu) * std::max(1, r.ctx->props.multiProcessorCount);
+    for (int inv = 0; inv < 2; ++inv)
+        for (int p = 0; p < RAG_PHASES; ++p) {
+            int per_cu = 0;
+            AFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rag_kernel(r.prec, p, inv ? 1 : -1),
+                                                                  RAG_THREADS, 0));
+            r.grid[inv][p] = std::max(1, per_cu) * std::max(1, r.ctx->props.multiProcessorCount);
+        }
     AFFT_HIP(hipMalloc(&r.scratch, (size_t)std::max<int64_t>(1, r.scratch_pts) * es));
     return AFFT_OK;
 }
@@ -244,7 +268,6 @@ static int ragged_build(afft_ragged_s& r) {
 static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, hipStream_t s) {
     void* rargs[] = {&r.counter};
     AFFT_HIP(hipLaunchKernel((const void*)&k_rag_reset, dim3(1), dim3(64), rargs, 0, s));
-    const void* fn = r.prec ? (const void*)&k_ragged<double> : (const void*)&k_ragged<float>;
     for (int p = 0; p < RAG_PHASES; ++p) {
         if (r.ntask[p] == 0) continue;
         RagArgs a{};
@@ -257,8 +280,8 @@ static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, 
         a.counter = r.counter + p;
         a.scratch = r.scratch;
         void* args[] = {&a};
-        const int grid = std::min(r.grid, r.ntask[p]);
-        AFFT_HIP(hipLaunchKernel(fn, dim3(grid), dim3(RAG_THREADS), args, 0, s));
+        const int grid = std::min(r.grid[dir > 0 ? 1 : 0][p], r.ntask[p]);
+        AFFT_HIP(hipLaunchKernel(rag_kernel(r.prec, p, dir), dim3(grid), dim3(RAG_THREADS), args, 0, s));
     }
     return AFFT_OK;
 }
