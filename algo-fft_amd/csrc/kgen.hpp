@@ -2,7 +2,9 @@
 // path; the reference's mixedradix.go:45-358 does the same recursion
 // depth-first on the CPU and sends 7-smooth sizes to Bluestein instead).
 //
-// One block holds G = 256 / TPT transforms, TPT threads each.  Every pass
+// One block of 256 threads (512 for transforms above 2048 points) holds
+// G = blockDim.x / TPT transforms, TPT threads each, about 8 points per
+// thread.  Every pass
 // reads its butterfly inputs from one LDS buffer (global memory for the first
 // pass), applies the per-pass twiddles, runs the radix-R codelet and writes
 // to the other LDS buffer (global memory for the last pass, whose output
@@ -12,7 +14,8 @@
 
 namespace afft {
 
-constexpr int GEN_THREADS = 256;
+constexpr int GEN_THREADS = 256;      // threads per block up to 8 * 256 points per transform
+constexpr int GEN_MAXTHREADS = 512;   // threads per block above
 constexpr int GEN_MAXPASS = 16;
 
 struct GArgs {
@@ -82,13 +85,13 @@ __device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS,
 // VAR_ROW / VAR_COL / VAR_4A / VAR_4B as in kernels.hpp.  ROW-like variants
 // lay a transform out contiguously in LDS, COL variants interleave G columns.
 template <typename T, int VAR, int DIR>
-__global__ void __launch_bounds__(GEN_THREADS) k_gen(GArgs g) {
+__global__ void __launch_bounds__(GEN_MAXTHREADS) k_gen(GArgs g) {
     extern __shared__ unsigned char smraw[];
     cx<T>* sm = (cx<T>*)smraw;
     const KArgs a = g.k;
     const int N = g.n;
     const int TPT = g.tpt;
-    const int G = GEN_THREADS / TPT;
+    const int G = blockDim.x / TPT;
     const int tid = threadIdx.x;
     constexpr bool ROWL = (VAR == VAR_ROW || VAR == VAR_4B);
     int c, t;

@@ -123,6 +123,31 @@ def sweep_leg(torch, A, ctx, stream, steps, x4096=None):
     return res
 
 
+def smooth_leg(torch, A, ctx, stream, steps):
+    """2/3/5/7-smooth lengths (the mixed-radix path) at batch 2^26/N, 512 MiB
+    per buffer: N = 1000 and 3000 and 6000 in one pass, 12000 and 100000
+    as four-step."""
+    res = []
+    g = torch.Generator(device="cuda:0").manual_seed(17)
+    for n in (1000, 3000, 6000, 12000, 100000):
+        batch = (1 << 26) // n
+        x = torch.complex(torch.rand(batch * n, device="cuda:0", generator=g) * 2 - 1,
+                          torch.rand(batch * n, device="cuda:0", generator=g) * 2 - 1)
+        y = torch.empty_like(x)
+        plan = A.Plan.from_desc(ctx, (n,), batch=batch)
+        ms = _time_plan(torch, plan, A, stream, y, x, steps)
+        rows = np.linspace(0, batch - 1, 4).astype(np.int64)
+        rel, mx = _accuracy(x.view(batch, n), y.view(batch, n), rows)
+        gbs = 16.0 * n * batch / (ms * 1e-3) / 1e9
+        res.append({"n": n, "batch": batch, "algorithm": plan.describe(), "ms": round(ms, 5),
+                    "gbs_algorithmic": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                    "rel_l2_vs_fp64": rel, "max_err_vs_fp64": mx})
+        plan.close()
+        del x, y
+    torch.cuda.empty_cache()
+    return res
+
+
 def nd_leg(torch, A, ctx, stream, steps):
     """N-D, real and FP64 workloads: float32 R2C 512^3, complex128 C2C
     8192^2 and complex128 Bluestein N=1,000,003 x 8, each forward, device
@@ -321,6 +346,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the precision sweep over N")
     ap.add_argument("--no-host", action="store_true", help="skip the host-memory end-to-end leg")
     ap.add_argument("--no-nd", action="store_true", help="skip the N-D / real / FP64 leg")
+    ap.add_argument("--no-smooth", action="store_true", help="skip the smooth-length leg")
     ap.add_argument("--no-latency", action="store_true", help="skip the small-transform latency probe")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -431,6 +457,8 @@ def main():
                                            x if (n == 4096 and batch == (1 << 26) // 4096) else None)
     del y
     torch.cuda.empty_cache()
+    if not args.no_smooth:
+        out["smooth_sizes"] = smooth_leg(torch, A, ctx, stream, max(5, args.steps // 5))
     if not args.no_nd:
         out["nd_real_fp64"] = nd_leg(torch, A, ctx, stream, max(3, args.steps // 10))
     if not args.no_latency:
