@@ -350,7 +350,7 @@ def main():
     ap.add_argument("--no-latency", action="store_true", help="skip the small-transform latency probe")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+        print(json.dumps({"error": "bench.py runs on one GPU; --gpus must be 1 "
                                    "(batched transforms shard along the batch with no exchange)"}))
         sys.exit(2)
 
