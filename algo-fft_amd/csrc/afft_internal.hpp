@@ -104,6 +104,11 @@ struct Plan {
     void* stage_out = nullptr;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
     struct HostPipe* pipe = nullptr;   // chunked host-slice execution (runtime.cpp)
+    // Workspace ordering across streams: the last stream that used tmp[] and
+    // an event recorded on it after that use.
+    bool ws_used = false;
+    hipStream_t ws_stream = nullptr;
+    hipEvent_t ws_done = nullptr;
     std::string algo;
     std::mutex mu;
     ~Plan();

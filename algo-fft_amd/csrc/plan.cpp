@@ -28,6 +28,7 @@ Plan::~Plan() {
     if (stage_in) (void)hipFree(stage_in);
     if (stage_out) (void)hipFree(stage_out);
     if (pipe) destroy_host_pipe(pipe);
+    if (ws_done) (void)hipEventDestroy(ws_done);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1284,10 +1285,29 @@ static int build_plan_steps(Plan& p) {
     return AFFT_OK;
 }
 
+// A plan's workspace (tmp[]) is shared by all its executions.  When one
+// execution is enqueued on a different stream than the previous one, the new
+// stream first waits for the previous use to finish, so a plan may be driven
+// from several streams (and host threads) without racing on its scratch.
+// Under stream capture the caller orders the graph's nodes and no event is
+// touched.
 int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s) {
     const int di = dir == AFFT_INVERSE ? 1 : 0;
+    bool ws = false;
     for (int t = 0; t < NTMP; ++t) {
-        if (p.tmp_bytes[t] && !p.tmp[t]) AFFT_HIP(hipMalloc(&p.tmp[t], p.tmp_bytes[t]));
+        if (!p.tmp_bytes[t]) continue;
+        ws = true;
+        if (!p.tmp[t]) AFFT_HIP(hipMalloc(&p.tmp[t], p.tmp_bytes[t]));
+    }
+    if (ws) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        AFFT_HIP(hipStreamIsCapturing(s, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            ws = false;
+        } else {
+            if (!p.ws_done) AFFT_HIP(hipEventCreateWithFlags(&p.ws_done, hipEventDisableTiming));
+            if (p.ws_used && p.ws_stream != s) AFFT_HIP(hipStreamWaitEvent(s, p.ws_done, 0));
+        }
     }
     void* base[ROLE_COUNT] = {const_cast<void*>(src), dst, p.tmp[0], p.tmp[1], p.tmp[2], p.tmp[3]};
     for (Step& st : p.steps[di]) {
@@ -1297,6 +1317,11 @@ int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s) {
         std::memcpy(st.arg.data() + st.out_field, &op, sizeof(void*));
         void* args[] = {st.arg.data()};
         AFFT_HIP(hipLaunchKernel(st.fn, st.grid, st.block, args, (size_t)st.lds_bytes, s));
+    }
+    if (ws) {
+        AFFT_HIP(hipEventRecord(p.ws_done, s));
+        p.ws_stream = s;
+        p.ws_used = true;
     }
     return AFFT_OK;
 }

@@ -11,9 +11,11 @@
  * complex64/complex128 slices; real values are float/double arrays.
  * Element counts are in complex elements unless stated otherwise.
  *
- * Threading: every object may be used from one thread at a time; distinct
- * plans may run concurrently on distinct streams.  The last error message is
- * thread-local.
+ * Threading: a plan's executions are serialised by a lock inside the plan,
+ * so one plan may be shared by several host threads (plan.go:14: plans are
+ * safe for concurrent use during transforms); other objects are used from one
+ * thread at a time.  Distinct plans (and clones) run concurrently on distinct
+ * streams.  The last error message is thread-local.
  */
 #ifndef ALGOFFT_HIP_H
 #define ALGOFFT_HIP_H
@@ -176,7 +178,10 @@ int afft_plan_forward(afft_plan p, void* dst, int64_t dst_len, const void* src, 
 int afft_plan_inverse(afft_plan p, void* dst, int64_t dst_len, const void* src, int64_t src_len);
 /* PlanImpl.Forward / Inverse with device-resident data (the Buffer case):
  * enqueues on `stream` (hipStream_t as void*, NULL = the plan's stream)
- * and returns without synchronising. */
+ * and returns without synchronising.  A plan with workspace that is enqueued
+ * on a different stream than its previous execution makes the new stream
+ * wait for that execution first (outside stream capture), so executions of
+ * one plan never overlap on its workspace; clones overlap freely. */
 int afft_plan_exec_device(afft_plan p, int direction, void* dst_dev, const void* src_dev, void* stream);
 int afft_plan_exec_buffers(afft_plan p, int direction, afft_buffer dst, afft_buffer src, afft_stream s);
 /* Waits for the work enqueued on the plan's own stream (exec calls made with
