@@ -33,6 +33,18 @@ __device__ __forceinline__ void st_out(cx<T>* p, cx<T> v, int nt) {
         *p = v;
     }
 }
+// j mod d for 0 <= j < 2^31 and d >= 1, given m = mod_magic(d) =
+// floor((2^32 - 1) / d).  q = mulhi(j, m) satisfies
+// j/d - j(1+d)/(d 2^32) < q <= j/d, and j(1+d)/(d 2^32) < 1, so q is
+// floor(j/d) or one less and a single correction yields the remainder.
+// Replaces the ~20-instruction integer division of a runtime modulus.
+__host__ __device__ constexpr unsigned mod_magic(unsigned d) { return 0xFFFFFFFFu / d; }
+__device__ __forceinline__ int fast_mod(int j, int d, unsigned m) {
+    const unsigned q = __umulhi((unsigned)j, m);
+    const int r = j - (int)(q * (unsigned)d);
+    return r >= d ? r - d : r;
+}
+
 template <typename T>
 __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
 template <typename T>

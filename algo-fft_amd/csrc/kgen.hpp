@@ -25,6 +25,7 @@ struct GArgs {
     int npass;
     int rad[GEN_MAXPASS];
     int ns[GEN_MAXPASS];
+    unsigned nsm[GEN_MAXPASS];   // mod_magic(ns[p])
     int twoff[GEN_MAXPASS];
 };
 
@@ -47,14 +48,14 @@ __device__ __forceinline__ void dft_rt(int R, cx<T>* v) {
 // the N/R in the pass; load(n) gives input point n, store(k, v) takes output
 // point k = (j/Ns)*Ns*R + (j mod Ns) + r*Ns.
 template <int R, int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, const cx<T>* __restrict__ tw, Load load,
-                                        Store store) {
+__device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, unsigned NSM, const cx<T>* __restrict__ tw,
+                                        Load load, Store store) {
     const int NR = N / R;
     for (int j = t; j < NR; j += TT) {
         cx<T> v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = load(j + r * NR);
-        const int k = j % NS;
+        const int k = fast_mod(j, NS, NSM);
         if (NS > 1) {
             const cx<T>* w = tw + k * (R - 1);
 #pragma unroll
@@ -68,16 +69,16 @@ __device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, const cx<T
 }
 
 template <int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS, const cx<T>* tw, Load load,
-                                            Store store) {
+__device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS, unsigned NSM, const cx<T>* tw,
+                                            Load load, Store store) {
     switch (R) {
-    case 2: rt_pass<2, DIR>(t, TT, N, NS, tw, load, store); break;
-    case 3: rt_pass<3, DIR>(t, TT, N, NS, tw, load, store); break;
-    case 4: rt_pass<4, DIR>(t, TT, N, NS, tw, load, store); break;
-    case 5: rt_pass<5, DIR>(t, TT, N, NS, tw, load, store); break;
-    case 7: rt_pass<7, DIR>(t, TT, N, NS, tw, load, store); break;
-    case 8: rt_pass<8, DIR>(t, TT, N, NS, tw, load, store); break;
-    case 16: rt_pass<16, DIR>(t, TT, N, NS, tw, load, store); break;
+    case 2: rt_pass<2, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 3: rt_pass<3, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 4: rt_pass<4, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 5: rt_pass<5, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 7: rt_pass<7, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 8: rt_pass<8, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 16: rt_pass<16, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
     default: break;
     }
 }
@@ -91,12 +92,13 @@ __global__ void __launch_bounds__(GEN_MAXTHREADS) k_gen(GArgs g) {
     const KArgs a = g.k;
     const int N = g.n;
     const int TPT = g.tpt;
-    const int G = blockDim.x / TPT;
+    const int lgt = __ffs(TPT) - 1;              // TPT and G are powers of two
+    const int G = blockDim.x >> lgt;
     const int tid = threadIdx.x;
     constexpr bool ROWL = (VAR == VAR_ROW || VAR == VAR_4B);
     int c, t;
-    if constexpr (ROWL) { c = tid / TPT; t = tid % TPT; }
-    else { c = tid % G; t = tid / G; }
+    if constexpr (ROWL) { c = tid >> lgt; t = tid & (TPT - 1); }
+    else { c = tid & (G - 1); t = tid >> (__ffs(G) - 1); }
     const long long cg = (long long)blockIdx.x * G + c;
     const bool act = cg < a.count;
     auto lidx = [&](int buf, int n) -> int {
@@ -112,6 +114,7 @@ __global__ void __launch_bounds__(GEN_MAXTHREADS) k_gen(GArgs g) {
     for (int p = 0; p < g.npass; ++p) {
         const int R = g.rad[p];
         const int NS = g.ns[p];
+        const unsigned NSM = g.nsm[p];
         const bool last = p == g.npass - 1;
         const cx<T>* twp = tw + g.twoff[p];
         const int rb = (p - 1) & 1, wb = p & 1;
@@ -135,18 +138,18 @@ __global__ void __launch_bounds__(GEN_MAXTHREADS) k_gen(GArgs g) {
             }
         };
         if (p == 0) {
-            if (last) rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load_g, store_o);
-            else rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load_g, store_s);
+            if (last) rt_pass_any<DIR>(R, t, TPT, N, NS, NSM, twp, load_g, store_o);
+            else rt_pass_any<DIR>(R, t, TPT, N, NS, NSM, twp, load_g, store_s);
         } else {
-            if (last) rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load_s, store_o);
-            else rt_pass_any<DIR>(R, t, TPT, N, NS, twp, load_s, store_s);
+            if (last) rt_pass_any<DIR>(R, t, TPT, N, NS, NSM, twp, load_s, store_o);
+            else rt_pass_any<DIR>(R, t, TPT, N, NS, NSM, twp, load_s, store_s);
         }
         __syncthreads();
     }
     if constexpr (VAR == VAR_4B) {
         // transposing store: lanes run along the transform index c
         const int fb = (g.npass - 1) & 1;
-        const int c2 = tid % G, t2 = tid / G;
+        const int c2 = tid & (G - 1), t2 = tid >> (__ffs(G) - 1);
         const long long cg2 = (long long)blockIdx.x * G + c2;
         if (cg2 < a.count) {
             for (int k = t2; k < N; k += TPT) st_out(out + cg2 + (long long)k * a.out_nd, sm[fb * G * N + c2 * N + k], a.nt);

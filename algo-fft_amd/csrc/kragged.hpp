@@ -45,6 +45,7 @@ struct RSched {
     int npass;
     int rad[RAG_SPASS];
     int ns[RAG_SPASS];
+    unsigned nsm[RAG_SPASS];     // mod_magic(ns[p])
     int twoff[RAG_SPASS];
     const void* tw;              // per-pass twiddles W_{Ns*R}^{k*r}
 };
@@ -104,8 +105,8 @@ __device__ __forceinline__ cx<T> rag_tw(const RJob& jb, long long i) {
 // writes output point (j/Ns)*Ns*R + j%Ns + r*Ns (store(k, v)).  Every thread
 // of the block calls this; `active` threads do the work.
 template <int R, int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, int NS, const cx<T>* __restrict__ tw,
-                                           Load load, Store store) {
+__device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, int NS, unsigned NSM,
+                                           const cx<T>* __restrict__ tw, Load load, Store store) {
     constexpr int MB = (16 + R - 1) / R;
     const int NR = N / R;
     cx<T> v[MB][R];
@@ -116,7 +117,7 @@ __device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, in
             if (j < NR) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[b][r] = load(j + r * NR);
-                const int k = j % NS;
+                const int k = fast_mod(j, NS, NSM);
                 if (NS > 1) {
                     const cx<T>* w = tw + k * (R - 1);
 #pragma unroll
@@ -132,7 +133,7 @@ __device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, in
         for (int b = 0; b < MB; ++b) {
             const int j = t + b * TT;
             if (j < NR) {
-                const int k = j % NS;
+                const int k = fast_mod(j, NS, NSM);
                 const int base = (j - k) * R + k;
 #pragma unroll
                 for (int r = 0; r < R; ++r) store(base + r * NS, v[b][r]);
@@ -142,16 +143,16 @@ __device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, in
 }
 
 template <int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass_ip_any(int R, bool active, int t, int TT, int N, int NS, const cx<T>* tw,
-                                               Load load, Store store) {
+__device__ __forceinline__ void rt_pass_ip_any(int R, bool active, int t, int TT, int N, int NS, unsigned NSM,
+                                               const cx<T>* tw, Load load, Store store) {
     switch (R) {
-    case 2: rt_pass_ip<2, DIR>(active, t, TT, N, NS, tw, load, store); break;
-    case 3: rt_pass_ip<3, DIR>(active, t, TT, N, NS, tw, load, store); break;
-    case 4: rt_pass_ip<4, DIR>(active, t, TT, N, NS, tw, load, store); break;
-    case 5: rt_pass_ip<5, DIR>(active, t, TT, N, NS, tw, load, store); break;
-    case 7: rt_pass_ip<7, DIR>(active, t, TT, N, NS, tw, load, store); break;
-    case 8: rt_pass_ip<8, DIR>(active, t, TT, N, NS, tw, load, store); break;
-    case 16: rt_pass_ip<16, DIR>(active, t, TT, N, NS, tw, load, store); break;
+    case 2: rt_pass_ip<2, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 3: rt_pass_ip<3, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 4: rt_pass_ip<4, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 5: rt_pass_ip<5, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 7: rt_pass_ip<7, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 8: rt_pass_ip<8, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 16: rt_pass_ip<16, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
     default: {   // radix 1: the copy of a one-point transform
         cx<T> v0 = mk<T>(0, 0);
         if (active && t < N) v0 = load(t);
@@ -165,11 +166,11 @@ __device__ __forceinline__ void rt_pass_ip_any(int R, bool active, int t, int TT
 // buf + c*P), group c of TPT threads on transform c, all passes in place;
 // the last pass hands its outputs to sink(c, k, v), which may write the
 // transform's own row.  Every thread of the block calls this; the buffer is
-// complete when it returns.
+// complete when it returns.  TPT is a power of two.
 template <typename T, int DIR, class Sink>
 __device__ __forceinline__ void rag_passes(const RSched& sc, int L, int P, int cnt, int TPT, cx<T>* buf, Sink sink) {
     const int tid = threadIdx.x;
-    const int grp = tid / TPT, t = tid - grp * TPT;
+    const int grp = tid >> (__ffs(TPT) - 1), t = tid & (TPT - 1);
     const bool active = grp < cnt;
     cx<T>* row = buf + grp * P;
     const cx<T>* __restrict__ tw = (const cx<T>*)sc.tw;
@@ -178,10 +179,95 @@ __device__ __forceinline__ void rag_passes(const RSched& sc, int L, int P, int c
         auto store = [&](int k, cx<T> v) { row[k] = v; };
         auto fin = [&](int k, cx<T> v) { sink(grp, k, v); };
         const int R = sc.rad[p];
-        if (p == sc.npass - 1) rt_pass_ip_any<DIR>(R, active, t, TPT, L, sc.ns[p], tw + sc.twoff[p], load, fin);
-        else rt_pass_ip_any<DIR>(R, active, t, TPT, L, sc.ns[p], tw + sc.twoff[p], load, store);
+        if (p == sc.npass - 1)
+            rt_pass_ip_any<DIR>(R, active, t, TPT, L, sc.ns[p], sc.nsm[p], tw + sc.twoff[p], load, fin);
+        else
+            rt_pass_ip_any<DIR>(R, active, t, TPT, L, sc.ns[p], sc.nsm[p], tw + sc.twoff[p], load, store);
         __syncthreads();
     }
+}
+
+// Staging between global memory and LDS.  A block moves at most RAG_PTS
+// points, RAG_ITERS steps per thread.  rag_staged issues the global loads
+// of a batch of steps before it uses the first one (about 128 bytes per
+// thread in flight: 16 complex64 or 8 complex128 points, half that when a
+// step loads two values), and no step needs an integer division.
+constexpr int RAG_ITERS = RAG_PTS / RAG_THREADS;
+
+// ld(s) returns step s's loaded value (its addresses must be valid for every
+// s < nsteps: callers clamp them); put(s, v) consumes it, in order of s.
+template <int BYTES, class Ld, class Put>
+__device__ __forceinline__ void rag_staged(int nsteps, Ld ld, Put put) {
+    constexpr int SB = 128 / BYTES < 1 ? 1 : (128 / BYTES > RAG_ITERS ? RAG_ITERS : 128 / BYTES);
+#pragma unroll
+    for (int h = 0; h < RAG_ITERS; h += SB) {
+        if (h >= nsteps) break;
+        decltype(ld(0)) v[SB];
+#pragma unroll
+        for (int s = 0; s < SB; ++s)
+            if (h + s < nsteps) v[s] = ld(h + s);
+#pragma unroll
+        for (int s = 0; s < SB; ++s)
+            if (h + s < nsteps) put(h + s, v[s]);
+    }
+}
+
+template <typename T>
+struct cx2 {
+    cx<T> a, b;
+};
+
+// Tiles whose contiguous global dimension is the sub-transform index c
+// (cnt <= G sub-transforms, G a power of two): lane c = tid mod G, line
+// l = tid / G + s * (RAG_THREADS / G) in step s; nsteps(L) steps cover L lines.
+struct ColLanes {
+    int c, l0, step;
+    __device__ explicit ColLanes(int G) {
+        const int lg = __ffs(G) - 1;
+        c = threadIdx.x & (G - 1);
+        l0 = threadIdx.x >> lg;
+        step = RAG_THREADS >> lg;
+    }
+    __device__ int line(int s) const { return l0 + s * step; }
+    __device__ int nsteps(int L) const { return (L + step - 1) / step; }
+};
+
+// Flat index i = tid + s * RAG_THREADS of rows of length L, as (row c,
+// element n), advanced by one step without a division.
+struct RowWalk {
+    int c, n, dq, dr, L;
+    __device__ explicit RowWalk(int len) : L(len) {
+        c = (int)threadIdx.x / L;
+        n = (int)threadIdx.x - c * L;
+        dq = RAG_THREADS / L;
+        dr = RAG_THREADS - dq * L;
+    }
+    __device__ void next() {
+        n += dr;
+        c += dq;
+        if (n >= L) {
+            n -= L;
+            ++c;
+        }
+    }
+};
+
+__device__ __forceinline__ int flat_steps(int total) { return (total + RAG_THREADS - 1) / RAG_THREADS; }
+
+// Contiguous run src[0 .. total) into LDS rows of pitch L + 1 (row c of the
+// run starts at src + c*L).
+template <typename T>
+__device__ __forceinline__ void rag_stage_rows(const cx<T>* __restrict__ src, int total, int L, cx<T>* lds) {
+    const int tid = threadIdx.x;
+    RowWalk w(L);
+    rag_staged<sizeof(cx<T>)>(
+        flat_steps(total), [&](int s) { return src[min(tid + s * RAG_THREADS, total - 1)]; },
+        [&](int s, cx<T> v) {
+            const int i = tid + s * RAG_THREADS;
+            if (i < total) lds[i + w.c] = v;
+            w.next();
+        });
+    __syncthreads();
 }
 
 // A whole job in one block: x staged into LDS with contiguous lanes, the
@@ -192,7 +278,12 @@ __device__ void rag_whole(const RagArgs a, const RJob& jb, cx<T>* lds) {
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
     cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
     const int m = jb.m, tid = threadIdx.x;
-    for (int i = tid; i < m; i += RAG_THREADS) lds[i] = x[i];
+    rag_staged<sizeof(cx<T>)>(
+        flat_steps(m), [&](int s) { return x[min(tid + s * RAG_THREADS, m - 1)]; },
+        [&](int s, cx<T> v) {
+            const int i = tid + s * RAG_THREADS;
+            if (i < m) lds[i] = v;
+        });
     __syncthreads();
     const T sc = DIR > 0 ? T(1) / T(m) : T(1);
     auto sink = [&](int, int k, cx<T> v) { y[k] = scl(v, sc); };
@@ -209,19 +300,34 @@ __device__ void rag_whole_blue(const RagArgs a, const RJob& jb, cx<T>* lds) {
     const bool inv = a.dir > 0;
     const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
     const cx<T>* __restrict__ filt = (const cx<T>*)jb.filt[inv ? 1 : 0];
-    for (int i = tid; i < m; i += RAG_THREADS) {
-        cx<T> v = mk<T>(0, 0);
-        if (i < n) v = inv ? cmulc(x[i], chirp[i]) : cmul(x[i], chirp[i]);
-        lds[i] = v;
-    }
+    rag_staged<sizeof(cx2<T>)>(
+        flat_steps(m),
+        [&](int s) {
+            const int i = min(tid + s * RAG_THREADS, n - 1);
+            return cx2<T>{x[i], chirp[i]};
+        },
+        [&](int s, cx2<T> v) {
+            const int i = tid + s * RAG_THREADS;
+            if (i < m) lds[i] = i < n ? (inv ? cmulc(v.a, v.b) : cmul(v.a, v.b)) : mk<T>(0, 0);
+        });
     __syncthreads();
     const T sc = inv ? T(1) / T(n) : T(1);
     auto sink = [&](int, int k, cx<T> v) { lds[k] = v; };
     rag_passes<T, -1>(jb.s[0], m, m, 1, RAG_THREADS, lds, sink);
-    for (int i = tid; i < m; i += RAG_THREADS) lds[i] = cmul(lds[i], filt[i]);
+    rag_staged<sizeof(cx<T>)>(
+        flat_steps(m), [&](int s) { return filt[min(tid + s * RAG_THREADS, m - 1)]; },
+        [&](int s, cx<T> w) {
+            const int i = tid + s * RAG_THREADS;
+            if (i < m) lds[i] = cmul(lds[i], w);
+        });
     __syncthreads();
     rag_passes<T, 1>(jb.s[0], m, m, 1, RAG_THREADS, lds, sink);
-    for (int i = tid; i < n; i += RAG_THREADS) y[i] = scl(inv ? cmulc(lds[i], chirp[i]) : cmul(lds[i], chirp[i]), sc);
+    rag_staged<sizeof(cx<T>)>(
+        flat_steps(n), [&](int s) { return chirp[min(tid + s * RAG_THREADS, n - 1)]; },
+        [&](int s, cx<T> w) {
+            const int i = tid + s * RAG_THREADS;
+            if (i < n) y[i] = scl(inv ? cmulc(lds[i], w) : cmul(lds[i], w), sc);
+        });
 }
 
 // Column task of a split job: columns c0 .. c0+cnt-1 of the m1 x m2 view
@@ -229,61 +335,69 @@ __device__ void rag_whole_blue(const RagArgs a, const RJob& jb, cx<T>* lds) {
 // S[k1*m2 + n2].  Bluestein jobs load chirp * x, zero past n.
 template <typename T, int SD>
 __device__ void rag_cols(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
-    const int L = jb.m1, P = L + 1, TPT = RAG_THREADS / jb.g_col;
-    const int cnt = tk.count, c0 = tk.first, m2 = jb.m2, tid = threadIdx.x;
+    const int L = jb.m1, P = L + 1, G = jb.g_col, TPT = RAG_THREADS / G;
+    const int cnt = tk.count, c0 = tk.first, m2 = jb.m2;
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
     const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
     cx<T>* __restrict__ S = (cx<T>*)a.scratch + jb.scr_off;
     const bool inv = a.dir > 0;
-#pragma unroll 2
-    for (int i = tid; i < cnt * L; i += RAG_THREADS) {
-        const int n1 = i / cnt, c = i - n1 * cnt;
-        const long long idx = (long long)m2 * n1 + c0 + c;
-        cx<T> v;
-        if (jb.blue) {
-            v = mk<T>(0, 0);
-            if (idx < jb.n) v = inv ? cmulc(x[idx], chirp[idx]) : cmul(x[idx], chirp[idx]);
-        } else {
-            v = x[idx];
-        }
-        lds[c * P + n1] = v;
+    const ColLanes ln(G);
+    const bool lane = ln.c < cnt;
+    const int cc = c0 + min(ln.c, cnt - 1);
+    const int ns = ln.nsteps(L);
+    if (jb.blue) {
+        const long long last = jb.n - 1;
+        rag_staged<sizeof(cx2<T>)>(
+            ns,
+            [&](int s) {
+                const long long idx = min((long long)m2 * min(ln.line(s), L - 1) + cc, last);
+                return cx2<T>{x[idx], chirp[idx]};
+            },
+            [&](int s, cx2<T> v) {
+                const int n1 = ln.line(s);
+                if (n1 < L && lane)
+                    lds[ln.c * P + n1] = (long long)m2 * n1 + cc <= last ? (inv ? cmulc(v.a, v.b) : cmul(v.a, v.b))
+                                                                          : mk<T>(0, 0);
+            });
+    } else {
+        rag_staged<sizeof(cx<T>)>(
+            ns, [&](int s) { return x[(long long)m2 * min(ln.line(s), L - 1) + cc]; },
+            [&](int s, cx<T> v) {
+                const int n1 = ln.line(s);
+                if (n1 < L && lane) lds[ln.c * P + n1] = v;
+            });
     }
     __syncthreads();
     auto sink = [&](int c, int k1, cx<T> v) { lds[c * P + k1] = v; };
     rag_passes<T, SD>(jb.s[0], L, P, cnt, TPT, lds, sink);
-#pragma unroll 2
-    for (int i = tid; i < cnt * L; i += RAG_THREADS) {
-        const int k1 = i / cnt, c = i - k1 * cnt;
-        S[(long long)k1 * m2 + c0 + c] = twmul<SD>(lds[c * P + k1], rag_tw<T>(jb, (long long)(c0 + c) * k1));
-    }
-}
-
-// Staging of rows r0 .. r0+cnt-1 of S (length L) into LDS rows of pitch P.
-template <typename T>
-__device__ __forceinline__ void rag_rows_in(const cx<T>* __restrict__ S, int r0, int cnt, int L, int P, cx<T>* lds) {
-#pragma unroll 2
-    for (int i = threadIdx.x; i < cnt * L; i += RAG_THREADS) {
-        const int c = i / L, n2 = i - c * L;
-        lds[c * P + n2] = S[(long long)(r0 + c) * L + n2];
-    }
-    __syncthreads();
+    rag_staged<sizeof(cx2<T>)>(
+        ns, [&](int s) { return rag_tw<T>(jb, (long long)cc * min(ln.line(s), L - 1)); },
+        [&](int s, cx<T> w) {
+            const int k1 = ln.line(s);
+            if (k1 < L && lane) S[(long long)k1 * m2 + cc] = twmul<SD>(lds[ln.c * P + k1], w);
+        });
 }
 
 // Row task of a split job: rows r0 .. r0+cnt-1 of S (length m2), FFT over n2
 // with sign SD, output X[k1 + m1*k2] (scaled 1/m for the inverse).
 template <typename T, int SD>
 __device__ void rag_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
-    const int L = jb.m2, P = L + 1, TPT = RAG_THREADS / jb.g_row;
-    const int cnt = tk.count, r0 = tk.first, m1 = jb.m1, tid = threadIdx.x;
-    rag_rows_in<T>((const cx<T>*)a.scratch + jb.scr_off, r0, cnt, L, P, lds);
+    const int L = jb.m2, P = L + 1, G = jb.g_row, TPT = RAG_THREADS / G;
+    const int cnt = tk.count, r0 = tk.first, m1 = jb.m1;
+    rag_stage_rows<T>((const cx<T>*)a.scratch + jb.scr_off + (long long)r0 * L, cnt * L, L, lds);
     auto sink = [&](int c, int k2, cx<T> v) { lds[c * P + k2] = v; };
     rag_passes<T, SD>(jb.s[1], L, P, cnt, TPT, lds, sink);
     const T sc = SD > 0 ? T(1) / T(jb.m) : T(1);
-    cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
-#pragma unroll 2
-    for (int i = tid; i < cnt * L; i += RAG_THREADS) {
-        const int k2 = i / cnt, c = i - k2 * cnt;
-        y[(r0 + c) + (long long)m1 * k2] = scl(lds[c * P + k2], sc);
+    cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off + r0;
+    const ColLanes ln(G);
+    if (ln.c < cnt) {
+        const int ns = ln.nsteps(L);
+#pragma unroll
+        for (int s = 0; s < RAG_ITERS; ++s) {
+            if (s >= ns) break;
+            const int k2 = ln.line(s);
+            if (k2 < L) y[ln.c + (long long)m1 * k2] = scl(lds[ln.c * P + k2], sc);
+        }
     }
 }
 
@@ -292,24 +406,38 @@ __device__ void rag_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>
 template <typename T>
 __device__ void rag_rows_blue(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
     const int L = jb.m2, P = L + 1, TPT = RAG_THREADS / jb.g_row;
-    const int cnt = tk.count, r0 = tk.first, tid = threadIdx.x;
-    cx<T>* __restrict__ S = (cx<T>*)a.scratch + jb.scr_off;
-    rag_rows_in<T>(S, r0, cnt, L, P, lds);
-    const cx<T>* __restrict__ filt = (const cx<T>*)jb.filt[a.dir > 0 ? 1 : 0];
+    const int cnt = tk.count, r0 = tk.first, tid = threadIdx.x, total = cnt * L;
+    cx<T>* __restrict__ S = (cx<T>*)a.scratch + jb.scr_off + (long long)r0 * L;
+    rag_stage_rows<T>(S, total, L, lds);
+    const cx<T>* __restrict__ filt = (const cx<T>*)jb.filt[a.dir > 0 ? 1 : 0] + (long long)r0 * L;
     auto sink = [&](int c, int k, cx<T> v) { lds[c * P + k] = v; };
     rag_passes<T, -1>(jb.s[1], L, P, cnt, TPT, lds, sink);
-#pragma unroll 2
-    for (int i = tid; i < cnt * L; i += RAG_THREADS) {
-        const int c = i / L, k2 = i - c * L;
-        lds[c * P + k2] = cmul(lds[c * P + k2], filt[(long long)(r0 + c) * L + k2]);
+    {
+        RowWalk rw(L);
+        rag_staged<sizeof(cx<T>)>(
+            flat_steps(total), [&](int s) { return filt[min(tid + s * RAG_THREADS, total - 1)]; },
+            [&](int s, cx<T> w) {
+                const int i = tid + s * RAG_THREADS;
+                if (i < total) lds[i + rw.c] = cmul(lds[i + rw.c], w);
+                rw.next();
+            });
     }
     __syncthreads();
     rag_passes<T, 1>(jb.s[1], L, P, cnt, TPT, lds, sink);
-#pragma unroll 2
-    for (int i = tid; i < cnt * L; i += RAG_THREADS) {
-        const int c = i / L, p = i - c * L;
-        S[(long long)(r0 + c) * L + p] = twmul<1>(lds[c * P + p], rag_tw<T>(jb, (long long)(r0 + c) * p));
-    }
+    RowWalk rw(L), rl(L);
+    rag_staged<sizeof(cx2<T>)>(
+        flat_steps(total),
+        [&](int s) {
+            // rows past the task's last one clamp to it
+            const cx<T> w = rag_tw<T>(jb, (long long)(r0 + min(rl.c, cnt - 1)) * rl.n);
+            rl.next();
+            return w;
+        },
+        [&](int s, cx<T> w) {
+            const int i = tid + s * RAG_THREADS;
+            if (i < total) S[i] = twmul<1>(lds[i + rw.c], w);
+            rw.next();
+        });
 }
 
 // Second column task of a split Bluestein job: columns p0 .. p0+cnt-1 of S,
@@ -317,30 +445,34 @@ __device__ void rag_rows_blue(const RagArgs a, const RJob& jb, const RTask& tk, 
 // the inverse transform), cropped to n.
 template <typename T>
 __device__ void rag_cols2(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
-    const int L = jb.m1, P = L + 1, TPT = RAG_THREADS / jb.g_col;
-    const int cnt = tk.count, p0 = tk.first, m2 = jb.m2, tid = threadIdx.x;
+    const int L = jb.m1, P = L + 1, G = jb.g_col, TPT = RAG_THREADS / G;
+    const int cnt = tk.count, p0 = tk.first, m2 = jb.m2;
     const cx<T>* __restrict__ S = (const cx<T>*)a.scratch + jb.scr_off;
     const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
     cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
-#pragma unroll 2
-    for (int i = tid; i < cnt * L; i += RAG_THREADS) {
-        const int k1 = i / cnt, c = i - k1 * cnt;
-        lds[c * P + k1] = S[(long long)k1 * m2 + p0 + c];
-    }
+    const ColLanes ln(G);
+    const bool lane = ln.c < cnt;
+    const int pc = p0 + min(ln.c, cnt - 1);
+    const int ns = ln.nsteps(L);
+    rag_staged<sizeof(cx<T>)>(
+        ns, [&](int s) { return S[(long long)min(ln.line(s), L - 1) * m2 + pc]; },
+        [&](int s, cx<T> v) {
+            const int k1 = ln.line(s);
+            if (k1 < L && lane) lds[ln.c * P + k1] = v;
+        });
     __syncthreads();
     auto sink = [&](int c, int q, cx<T> v) { lds[c * P + q] = v; };
     rag_passes<T, 1>(jb.s[0], L, P, cnt, TPT, lds, sink);
     const bool inv = a.dir > 0;
     const T sc = inv ? T(1) / T(jb.n) : T(1);
-#pragma unroll 2
-    for (int i = tid; i < cnt * L; i += RAG_THREADS) {
-        const int q = i / cnt, c = i - q * cnt;
-        const long long n = p0 + c + (long long)m2 * q;
-        if (n < jb.n) {
-            const cx<T> v = lds[c * P + q];
-            y[n] = scl(inv ? cmulc(v, chirp[n]) : cmul(v, chirp[n]), sc);
-        }
-    }
+    const long long last = jb.n - 1;
+    rag_staged<sizeof(cx<T>)>(
+        ns, [&](int s) { return chirp[min(pc + (long long)m2 * min(ln.line(s), L - 1), last)]; },
+        [&](int s, cx<T> w) {
+            const int q = ln.line(s);
+            const long long n = pc + (long long)m2 * q;
+            if (q < L && lane && n <= last) y[n] = scl(inv ? cmulc(lds[ln.c * P + q], w) : cmul(lds[ln.c * P + q], w), sc);
+        });
 }
 
 // The persistent loop shared by the four task kernels.

@@ -473,12 +473,14 @@ static int gen_args(Plan& p, int64_t n, bool rows, GArgs& g, int* tpt, int* lds,
     int64_t ns = rad[0];
     g.npass = (int)rad.size();
     g.ns[0] = 1;
+    g.nsm[0] = mod_magic(1);
     g.twoff[0] = 0;
     g.rad[0] = rad[0];
     for (size_t i = 1; i < rad.size(); ++i) {
         const int R = rad[i];
         g.rad[i] = R;
         g.ns[i] = (int)ns;
+        g.nsm[i] = mod_magic((unsigned)ns);
         g.twoff[i] = (int)(tw.size() / 2);
         for (int64_t kk = 0; kk < ns; ++kk)
             for (int r = 1; r < R; ++r) {

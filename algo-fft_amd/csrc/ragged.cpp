@@ -73,6 +73,7 @@ static int schedule_into(Plan& tab, int prec, int64_t m, RSched& j, std::map<int
         j.npass = 1;
         j.rad[0] = 1;
         j.ns[0] = 1;
+        j.nsm[0] = mod_magic(1);
         cache[m] = j;
         return AFFT_OK;
     }
@@ -86,6 +87,7 @@ static int schedule_into(Plan& tab, int prec, int64_t m, RSched& j, std::map<int
     for (size_t i = 0; i < rad.size(); ++i) {
         j.rad[i] = rad[i];
         j.ns[i] = (int)ns;
+        j.nsm[i] = mod_magic((unsigned)ns);
         j.twoff[i] = off;
         if (i > 0) {
             for (int64_t k = 0; k < ns; ++k)
