@@ -52,10 +52,24 @@ struct RSched {
 
 enum { RT_WHOLE = 0, RT_COLS = 1, RT_ROWS = 2, RT_COLS2 = 3 };
 
-// Launches, in order: columns of plain jobs, columns of Bluestein jobs, rows
-// of plain jobs, rows of Bluestein jobs, second columns, whole plain jobs,
-// whole Bluestein jobs.
-enum { RL_COLS = 0, RL_COLS_BLUE, RL_ROWS, RL_ROWS_BLUE, RL_COLS2, RL_WHOLE, RL_WHOLE_BLUE, RL_COUNT };
+// Launches, in order: columns of plain split jobs (generic passes, then the
+// power-of-two cores), columns of Bluestein jobs, rows likewise, second
+// columns of Bluestein jobs, whole plain jobs, whole Bluestein jobs.
+enum {
+    RL_COLS = 0,
+    RL_COLS_P2,
+    RL_COLS_BLUE,
+    RL_ROWS,
+    RL_ROWS_P2,
+    RL_ROWS_BLUE,
+    RL_COLS2,
+    RL_WHOLE,
+    RL_WHOLE_BLUE,
+    RL_COUNT
+};
+constexpr int RAG_COUNTERS = 64;     // queue counters (one per launch) cleared by k_rag_reset
+constexpr int RAG_P2_COL_MIN = 64, RAG_P2_COL_MAX = 1024;    // power-of-two column lengths m1
+constexpr int RAG_P2_ROW_MIN = 128, RAG_P2_ROW_MAX = 2048;   // power-of-two row lengths m2
 
 struct RJob {
     long long in_off, out_off;   // element offsets of the job's input and output
@@ -330,43 +344,25 @@ __device__ void rag_whole_blue(const RagArgs a, const RJob& jb, cx<T>* lds) {
         });
 }
 
-// Column task of a split job: columns c0 .. c0+cnt-1 of the m1 x m2 view
-// x[m2*n1 + n2], FFT over n1 with sign SD, times W_m^{SD * n2 * k1}, into
-// S[k1*m2 + n2].  Bluestein jobs load chirp * x, zero past n.
+// Column task of a split job with generic passes: columns c0 .. c0+cnt-1 of
+// the m1 x m2 view x[m2*n1 + n2], FFT over n1 with sign SD, times
+// W_m^{SD * n2 * k1}, into S[k1*m2 + n2].
 template <typename T, int SD>
 __device__ void rag_cols(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
     const int L = jb.m1, P = L + 1, G = jb.g_col, TPT = RAG_THREADS / G;
     const int cnt = tk.count, c0 = tk.first, m2 = jb.m2;
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
-    const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
     cx<T>* __restrict__ S = (cx<T>*)a.scratch + jb.scr_off;
-    const bool inv = a.dir > 0;
     const ColLanes ln(G);
     const bool lane = ln.c < cnt;
     const int cc = c0 + min(ln.c, cnt - 1);
     const int ns = ln.nsteps(L);
-    if (jb.blue) {
-        const long long last = jb.n - 1;
-        rag_staged<sizeof(cx2<T>)>(
-            ns,
-            [&](int s) {
-                const long long idx = min((long long)m2 * min(ln.line(s), L - 1) + cc, last);
-                return cx2<T>{x[idx], chirp[idx]};
-            },
-            [&](int s, cx2<T> v) {
-                const int n1 = ln.line(s);
-                if (n1 < L && lane)
-                    lds[ln.c * P + n1] = (long long)m2 * n1 + cc <= last ? (inv ? cmulc(v.a, v.b) : cmul(v.a, v.b))
-                                                                          : mk<T>(0, 0);
-            });
-    } else {
-        rag_staged<sizeof(cx<T>)>(
-            ns, [&](int s) { return x[(long long)m2 * min(ln.line(s), L - 1) + cc]; },
-            [&](int s, cx<T> v) {
-                const int n1 = ln.line(s);
-                if (n1 < L && lane) lds[ln.c * P + n1] = v;
-            });
-    }
+    rag_staged<sizeof(cx<T>)>(
+        ns, [&](int s) { return x[(long long)m2 * min(ln.line(s), L - 1) + cc]; },
+        [&](int s, cx<T> v) {
+            const int n1 = ln.line(s);
+            if (n1 < L && lane) lds[ln.c * P + n1] = v;
+        });
     __syncthreads();
     auto sink = [&](int c, int k1, cx<T> v) { lds[c * P + k1] = v; };
     rag_passes<T, SD>(jb.s[0], L, P, cnt, TPT, lds, sink);
@@ -401,81 +397,153 @@ __device__ void rag_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>
     }
 }
 
-// Row task of a split Bluestein job: forward FFT, filter product, inverse
-// FFT, times W_m^{-k1*p}, back into the same row of S.
+// ---- power-of-two sub-transforms on the compile-time cores -------------------
+// Split jobs whose sub-transform lengths are powers of two (all plain
+// power-of-two jobs above RAG_WHOLE, and every Bluestein job, whose m is its
+// power-of-two padding; m <= 2^21 gives m1 in [64, 1024], m2 in [128, 2048])
+// run their column and row tasks on the radix-16 register cores of the
+// single-size kernels: 16 points per thread, L/16 threads per sub-transform,
+// G = RAG_PTS/L sub-transforms per task (the host cuts every such job into
+// tasks of exactly G), global loads straight into registers.  Each length L
+// is its own kernel and launch: one kernel holding several lengths would need
+// the registers of all of them.  The twiddles come from the job's runtime
+// schedule, whose table has the layout of the matching Sched (the remainder
+// radix first, no twiddles in pass 0).
+
+template <int L>
+struct P2Sched;
+template <> struct P2Sched<64> { using S = Sched<4, 16>; };
+template <> struct P2Sched<128> { using S = Sched<8, 16>; };
+template <> struct P2Sched<256> { using S = Sched<16, 16>; };
+template <> struct P2Sched<512> { using S = Sched<2, 16, 16>; };
+template <> struct P2Sched<1024> { using S = Sched<4, 16, 16>; };
+template <> struct P2Sched<2048> { using S = Sched<8, 16, 16>; };
+template <> struct P2Sched<4096> { using S = Sched<16, 16, 16>; };
+
+// LDS of the row tasks: G padded slabs (the column tasks need RAG_PTS).
 template <typename T>
-__device__ void rag_rows_blue(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
-    const int L = jb.m2, P = L + 1, TPT = RAG_THREADS / jb.g_row;
-    const int cnt = tk.count, r0 = tk.first, tid = threadIdx.x, total = cnt * L;
-    cx<T>* __restrict__ S = (cx<T>*)a.scratch + jb.scr_off + (long long)r0 * L;
-    rag_stage_rows<T>(S, total, L, lds);
-    const cx<T>* __restrict__ filt = (const cx<T>*)jb.filt[a.dir > 0 ? 1 : 0] + (long long)r0 * L;
-    auto sink = [&](int c, int k, cx<T> v) { lds[c * P + k] = v; };
-    rag_passes<T, -1>(jb.s[1], L, P, cnt, TPT, lds, sink);
-    {
-        RowWalk rw(L);
-        rag_staged<sizeof(cx<T>)>(
-            flat_steps(total), [&](int s) { return filt[min(tid + s * RAG_THREADS, total - 1)]; },
-            [&](int s, cx<T> w) {
-                const int i = tid + s * RAG_THREADS;
-                if (i < total) lds[i + rw.c] = cmul(lds[i + rw.c], w);
-                rw.next();
-            });
+struct RagP2 {
+    static constexpr int BUF = RAG_PTS + (RAG_PTS >> PadFor<T, 64>::value) + RAG_PTS / 64;
+};
+
+// Column task: columns c0 .. c0+G-1 of x[m2*n1 + n2] (chirp * x zero-padded
+// past n for Bluestein jobs), FFT over n1 with sign SD, times
+// W_m^{SD * n2 * k1}, into S[k1*m2 + n2].
+template <typename T, int L, int SD, bool BLUE>
+__device__ __forceinline__ void p2_cols(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
+    using S = typename P2Sched<L>::S;
+    constexpr int E = 16, TT = L / E, G = RAG_PTS / L;
+    const int tid = threadIdx.x, c = tid % G, t = tid / G;
+    const long long col = tk.first + c, m2 = jb.m2;
+    const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
+    cx<T> v[E];
+    if constexpr (BLUE) {
+        const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
+        const bool inv = a.dir > 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const long long idx = m2 * (t + TT * e) + col;
+            v[e] = mk<T>(0, 0);
+            if (idx < jb.n) v[e] = inv ? cmulc(x[idx], chirp[idx]) : cmul(x[idx], chirp[idx]);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = x[m2 * (t + TT * e) + col];
     }
-    __syncthreads();
-    rag_passes<T, 1>(jb.s[1], L, P, cnt, TPT, lds, sink);
-    RowWalk rw(L), rl(L);
-    rag_staged<sizeof(cx2<T>)>(
-        flat_steps(total),
-        [&](int s) {
-            // rows past the task's last one clamp to it
-            const cx<T> w = rag_tw<T>(jb, (long long)(r0 + min(rl.c, cnt - 1)) * rl.n);
-            rl.next();
-            return w;
-        },
-        [&](int s, cx<T> w) {
-            const int i = tid + s * RAG_THREADS;
-            if (i < total) S[i] = twmul<1>(lds[i + rw.c], w);
-            rw.next();
-        });
+    core<T, S, E, SD>(v, lds, t, (const cx<T>*)jb.s[0].tw, LdsCol<G>(c));
+    cx<T>* __restrict__ sp = (cx<T>*)a.scratch + jb.scr_off + col;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k1 = t + TT * e;
+        sp[m2 * k1] = twmul<SD>(v[e], rag_tw<T>(jb, col * k1));
+    }
 }
 
-// Second column task of a split Bluestein job: columns p0 .. p0+cnt-1 of S,
-// inverse FFT over k1, output z[p + m2*q] times the post-chirp (and 1/n for
-// the inverse transform), cropped to n.
-template <typename T>
-__device__ void rag_cols2(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
-    const int L = jb.m1, P = L + 1, G = jb.g_col, TPT = RAG_THREADS / G;
-    const int cnt = tk.count, p0 = tk.first, m2 = jb.m2;
-    const cx<T>* __restrict__ S = (const cx<T>*)a.scratch + jb.scr_off;
-    const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
-    cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
-    const ColLanes ln(G);
-    const bool lane = ln.c < cnt;
-    const int pc = p0 + min(ln.c, cnt - 1);
-    const int ns = ln.nsteps(L);
-    rag_staged<sizeof(cx<T>)>(
-        ns, [&](int s) { return S[(long long)min(ln.line(s), L - 1) * m2 + pc]; },
-        [&](int s, cx<T> v) {
-            const int k1 = ln.line(s);
-            if (k1 < L && lane) lds[ln.c * P + k1] = v;
-        });
+// The G rows r0 .. r0+G-1 of S, one per group of L/16 threads.
+template <typename T, int L>
+__device__ __forceinline__ void p2_row_load(const cx<T>* __restrict__ row, int t, cx<T> (&v)[16]) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = row[t + (L / 16) * e];
+}
+
+// Row task: rows r0 .. r0+G-1 of S, FFT over n2 with sign SD, output
+// X[k1 + m1*k2] (scaled 1/m for the inverse), stored through LDS so that the
+// lanes of a store run along k1.
+template <typename T, int L, int SD>
+__device__ __forceinline__ void p2_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
+    using S = typename P2Sched<L>::S;
+    constexpr int E = 16, TT = L / E, G = RAG_PTS / L, PADSH = PadFor<T, L>::value;
+    const int tid = threadIdx.x, c = tid / TT, t = tid % TT;
+    const int r0 = tk.first;
+    cx<T> v[E];
+    p2_row_load<T, L>((const cx<T>*)a.scratch + jb.scr_off + (long long)(r0 + c) * L, t, v);
+    const LdsRow<L, PADSH> li(c);
+    core<T, S, E, SD>(v, lds, t, (const cx<T>*)jb.s[1].tw, li);
     __syncthreads();
-    auto sink = [&](int c, int q, cx<T> v) { lds[c * P + q] = v; };
-    rag_passes<T, 1>(jb.s[0], L, P, cnt, TPT, lds, sink);
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[li(t + TT * e)] = v[e];
+    __syncthreads();
+    const int c2 = tid % G, t2 = tid / G;
+    const LdsRow<L, PADSH> li2(c2);
+    const T sc = SD > 0 ? T(1) / T(jb.m) : T(1);
+    cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off + r0 + c2;
+    const long long m1 = jb.m1;
+#pragma unroll
+    for (int e = 0; e < E; ++e) y[m1 * (t2 + TT * e)] = scl(lds[li2(t2 + TT * e)], sc);
+}
+
+// Row task of a Bluestein job: forward FFT, filter product (the filter is
+// stored row-major as [k1][k2]), inverse FFT, times W_m^{-k1*p}, back into
+// the same rows of S.
+template <typename T, int L>
+__device__ __forceinline__ void p2_rows_blue(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
+    using S = typename P2Sched<L>::S;
+    constexpr int E = 16, TT = L / E, PADSH = PadFor<T, L>::value;
+    const int tid = threadIdx.x, c = tid / TT, t = tid % TT;
+    const int k1 = tk.first + c;
+    cx<T>* __restrict__ row = (cx<T>*)a.scratch + jb.scr_off + (long long)k1 * L;
+    const cx<T>* __restrict__ frow = (const cx<T>*)jb.filt[a.dir > 0 ? 1 : 0] + (long long)k1 * L;
+    const cx<T>* __restrict__ tw = (const cx<T>*)jb.s[1].tw;
+    cx<T> v[E];
+    p2_row_load<T, L>(row, t, v);
+    const LdsRow<L, PADSH> li(c);
+    core<T, S, E, -1>(v, lds, t, tw, li);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cmul(v[e], frow[t + TT * e]);
+    core<T, S, E, 1, true>(v, lds, t, tw, li);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int p = t + TT * e;
+        row[p] = twmul<1>(v[e], rag_tw<T>(jb, (long long)k1 * p));
+    }
+}
+
+// Second column task of a Bluestein job: columns p0 .. p0+G-1 of S, inverse
+// FFT over k1, output z[p + m2*q] times the post-chirp (and 1/n for the
+// inverse transform), cropped to n.
+template <typename T, int L>
+__device__ __forceinline__ void p2_cols2(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
+    using S = typename P2Sched<L>::S;
+    constexpr int E = 16, TT = L / E, G = RAG_PTS / L;
+    const int tid = threadIdx.x, c = tid % G, t = tid / G;
+    const long long p = tk.first + c, m2 = jb.m2;
+    const cx<T>* __restrict__ sp = (const cx<T>*)a.scratch + jb.scr_off + p;
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = sp[m2 * (t + TT * e)];
+    core<T, S, E, 1>(v, lds, t, (const cx<T>*)jb.s[0].tw, LdsCol<G>(c));
     const bool inv = a.dir > 0;
     const T sc = inv ? T(1) / T(jb.n) : T(1);
-    const long long last = jb.n - 1;
-    rag_staged<sizeof(cx<T>)>(
-        ns, [&](int s) { return chirp[min(pc + (long long)m2 * min(ln.line(s), L - 1), last)]; },
-        [&](int s, cx<T> w) {
-            const int q = ln.line(s);
-            const long long n = pc + (long long)m2 * q;
-            if (q < L && lane && n <= last) y[n] = scl(inv ? cmulc(lds[ln.c * P + q], w) : cmul(lds[ln.c * P + q], w), sc);
-        });
+    const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
+    cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const long long n = p + m2 * (t + TT * e);
+        if (n < jb.n) y[n] = scl(inv ? cmulc(v[e], chirp[n]) : cmul(v[e], chirp[n]), sc);
+    }
 }
 
-// The persistent loop shared by the four task kernels.
+// The persistent loop shared by the task kernels.
 template <class Run>
 __device__ __forceinline__ void rag_queue(const RagArgs a, Run run) {
     __shared__ int s_task;
@@ -491,11 +559,17 @@ __device__ __forceinline__ void rag_queue(const RagArgs a, Run run) {
     }
 }
 
-// The kernels of the seven launches (RL_* in launch order).
+// The kernels of the launches (RL_* in launch order).
 template <typename T, int SD>
 __global__ void __launch_bounds__(RAG_THREADS) k_rag_cols(RagArgs a) {
     __shared__ cx<T> lds[RAG_BUF];
     rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_cols<T, SD>(a, jb, tk, lds); });
+}
+
+template <typename T, int L, int SD, bool BLUE>
+__global__ void __launch_bounds__(RAG_THREADS) k_rag_cols_p2(RagArgs a) {
+    __shared__ cx<T> lds[RAG_PTS];
+    rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_cols<T, L, SD, BLUE>(a, jb, tk, lds); });
 }
 
 template <typename T, int SD>
@@ -504,16 +578,22 @@ __global__ void __launch_bounds__(RAG_THREADS) k_rag_rows(RagArgs a) {
     rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_rows<T, SD>(a, jb, tk, lds); });
 }
 
-template <typename T>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_rows_blue(RagArgs a) {
-    __shared__ cx<T> lds[RAG_BUF];
-    rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_rows_blue<T>(a, jb, tk, lds); });
+template <typename T, int L, int SD>
+__global__ void __launch_bounds__(RAG_THREADS) k_rag_rows_p2(RagArgs a) {
+    __shared__ cx<T> lds[RagP2<T>::BUF];
+    rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_rows<T, L, SD>(a, jb, tk, lds); });
 }
 
-template <typename T>
+template <typename T, int L>
+__global__ void __launch_bounds__(RAG_THREADS) k_rag_rows_blue(RagArgs a) {
+    __shared__ cx<T> lds[RagP2<T>::BUF];
+    rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_rows_blue<T, L>(a, jb, tk, lds); });
+}
+
+template <typename T, int L>
 __global__ void __launch_bounds__(RAG_THREADS) k_rag_cols2(RagArgs a) {
-    __shared__ cx<T> lds[RAG_BUF];
-    rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_cols2<T>(a, jb, tk, lds); });
+    __shared__ cx<T> lds[RAG_PTS];
+    rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_cols2<T, L>(a, jb, tk, lds); });
 }
 
 template <typename T, int DIR>
@@ -528,10 +608,5 @@ __global__ void __launch_bounds__(RAG_THREADS) k_rag_whole_blue(RagArgs a) {
     rag_queue(a, [=](const RJob& jb, const RTask&) { rag_whole_blue<T>(a, jb, lds); });
 }
 
-// Clears the launch counters (a kernel, so the launches can be captured in a
-// graph and replayed).
-__global__ void k_rag_reset(int* counter) {
-    if (threadIdx.x < 8) counter[threadIdx.x] = 0;
-}
 
 }  // namespace afft

@@ -5,7 +5,8 @@
 //   afft_ragged_exec        the persistent work-queue kernels, one launch per
 //                           task type (column tasks, row tasks, the
 //                           Bluestein jobs' second column tasks, whole jobs;
-//                           plain and Bluestein jobs apart), after one
+//                           plain and Bluestein jobs apart; power-of-two
+//                           sub-transforms per length), after one
 //                           counter-reset kernel
 //   afft_ragged_exec_graph  the same launches captured once in a hipGraph
 //                           and replayed as one graph launch
@@ -23,7 +24,43 @@
 
 using namespace afft;
 
-constexpr int RAG_PHASES = RL_COUNT;
+namespace afft {
+const void* rag_p2_kernel(int prec, int kind, int L, int dir);   // ragged_p2.hip
+}
+
+namespace afft {
+// Clears the launch counters (a kernel, so the launches can be captured in a
+// graph and replayed).
+__global__ void k_rag_reset(int* counter) {
+    if (threadIdx.x < RAG_COUNTERS) counter[threadIdx.x] = 0;
+}
+}  // namespace afft
+
+// The launches form independent chains (the tasks of a job depend only on
+// earlier launches of the same chain): generic smooth splits, power-of-two
+// splits, Bluestein splits, whole jobs.
+constexpr int RAG_BRANCHES = 4;
+static int rag_branch(int kind) {
+    switch (kind) {
+    case RL_COLS:
+    case RL_ROWS: return 0;
+    case RL_COLS_P2:
+    case RL_ROWS_P2: return 1;
+    case RL_COLS_BLUE:
+    case RL_ROWS_BLUE:
+    case RL_COLS2: return 2;
+    default: return 3;
+    }
+}
+
+// One launch of the batch: a task type (RL_*), the sub-transform length of
+// the power-of-two types (0 for the others), its slice of the task array and
+// its persistent grid per direction.
+struct RagLaunch {
+    int kind = 0, L = 0;
+    int ntask = 0, task_off = 0;
+    int grid[2] = {0, 0};   // [inverse]
+};
 
 struct afft_ragged_s {
     afft_context ctx = nullptr;
@@ -36,13 +73,21 @@ struct afft_ragged_s {
     RJob* jobs_dev = nullptr;
     RTask* tasks_dev = nullptr;
     int njobs = 0;
-    int ntask[RAG_PHASES] = {};
-    int task_off[RAG_PHASES] = {};
-    int grid[2][RAG_PHASES] = {};       // [inverse][launch]
+    std::vector<RagLaunch> launches;    // in launch order
     int* counter = nullptr;
     void* scratch = nullptr;
     int64_t scratch_pts = 0;
     hipStream_t stream = nullptr;
+    // Independent chains of launches run as concurrent branches: the caller's
+    // stream takes branch 0, these streams the others; events fork and join.
+    hipStream_t branch[RAG_BRANCHES - 1] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[RAG_BRANCHES - 1] = {};
+    // Executions share the counters and the scratch array: one enqueued on
+    // another stream than the previous one waits for it first.
+    hipEvent_t done = nullptr;
+    hipStream_t last = nullptr;
+    bool ran = false;
     std::map<std::tuple<int, void*, const void*, void*>, hipGraphExec_t> graphs;
     std::map<int64_t, afft_plan> per_size;
     std::mutex mu;
@@ -58,6 +103,30 @@ afft_ragged_s::~afft_ragged_s() {
     if (counter) (void)hipFree(counter);
     if (scratch) (void)hipFree(scratch);
     if (stream) (void)hipStreamDestroy(stream);
+    for (int b = 0; b < RAG_BRANCHES - 1; ++b) {
+        if (branch[b]) (void)hipStreamDestroy(branch[b]);
+        if (join[b]) (void)hipEventDestroy(join[b]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (done) (void)hipEventDestroy(done);
+}
+
+// Orders an execution on s after the previous one (outside stream capture,
+// where the caller orders the graph's nodes).
+static int ragged_order_begin(afft_ragged_s& r, hipStream_t s, bool* track) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    AFFT_HIP(hipStreamIsCapturing(s, &cs));
+    *track = cs == hipStreamCaptureStatusNone;
+    if (*track && r.ran && r.last != s) AFFT_HIP(hipStreamWaitEvent(s, r.done, 0));
+    return AFFT_OK;
+}
+
+static int ragged_order_end(afft_ragged_s& r, hipStream_t s, bool track) {
+    if (!track) return AFFT_OK;
+    AFFT_HIP(hipEventRecord(r.done, s));
+    r.last = s;
+    r.ran = true;
+    return AFFT_OK;
 }
 
 // Radix schedule and per-pass twiddles of length m (m <= RAG_WHOLE), shared
@@ -133,23 +202,33 @@ static int pow2_floor(int64_t v) {
     return g;
 }
 
-// The persistent kernel of launch p (RL_*) for direction dir.
+// The persistent kernel of a launch for direction dir.
 template <typename T>
-static const void* rag_kernel_t(int p, int dir) {
-    switch (p) {
+static const void* rag_kernel_t(int kind, int dir) {
+    switch (kind) {
     case RL_COLS: return dir > 0 ? (const void*)&k_rag_cols<T, 1> : (const void*)&k_rag_cols<T, -1>;
-    case RL_COLS_BLUE: return (const void*)&k_rag_cols<T, -1>;
     case RL_ROWS: return dir > 0 ? (const void*)&k_rag_rows<T, 1> : (const void*)&k_rag_rows<T, -1>;
-    case RL_ROWS_BLUE: return (const void*)&k_rag_rows_blue<T>;
-    case RL_COLS2: return (const void*)&k_rag_cols2<T>;
     case RL_WHOLE: return dir > 0 ? (const void*)&k_rag_whole<T, 1> : (const void*)&k_rag_whole<T, -1>;
-    default: return (const void*)&k_rag_whole_blue<T>;
+    case RL_WHOLE_BLUE: return (const void*)&k_rag_whole_blue<T>;
+    default: return nullptr;
     }
 }
 
-static const void* rag_kernel(int prec, int p, int dir) {
-    return prec ? rag_kernel_t<double>(p, dir) : rag_kernel_t<float>(p, dir);
+static const void* rag_kernel(int prec, const RagLaunch& l, int dir) {
+    if (l.L) return rag_p2_kernel(prec, l.kind, l.L, dir);
+    return prec ? rag_kernel_t<double>(l.kind, dir) : rag_kernel_t<float>(l.kind, dir);
 }
+
+// A split runs on the power-of-two cores when both lengths are powers of two
+// in the ranges those kernels are built for and the tasks hold RAG_PTS/L
+// sub-transforms each.
+static bool p2_split(int64_t m1, int64_t m2, int g_col, int g_row) {
+    auto pow2 = [](int64_t L) { return L > 0 && (L & (L - 1)) == 0; };
+    return pow2(m1) && pow2(m2) && m1 >= RAG_P2_COL_MIN && m1 <= RAG_P2_COL_MAX && m2 >= RAG_P2_ROW_MIN &&
+           m2 <= RAG_P2_ROW_MAX && g_col == RAG_PTS / m1 && g_row == RAG_PTS / m2 && m2 % g_col == 0 &&
+           m1 % g_row == 0;
+}
+
 
 static int ragged_build(afft_ragged_s& r) {
     const int es = r.prec ? 16 : 8;
@@ -159,7 +238,9 @@ static int ragged_build(afft_ragged_s& r) {
     struct TwSplit { int shift; const void* lo; const void* hi; };
     std::map<int64_t, TwSplit> tcache;
     struct Pending { RTask t; double cost; };
-    std::vector<Pending> ph[RAG_PHASES];
+    // tasks per launch, keyed by (task type, sub-transform length): the
+    // map's order is the launch order
+    std::map<std::pair<int, int>, std::vector<Pending>> ph;
     int64_t off = 0;
     r.offs.resize(r.sizes.size());
     r.scratch_pts = 0;
@@ -224,67 +305,98 @@ static int ragged_build(afft_ragged_s& r) {
         const int job = (int)i;
         if (whole) {
             const double c = (blue ? 2.0 : 1.0) * m * std::log2(std::max<double>(2.0, (double)m));
-            ph[blue ? RL_WHOLE_BLUE : RL_WHOLE].push_back({{job, RT_WHOLE, 0, 1}, c});
+            ph[{blue ? RL_WHOLE_BLUE : RL_WHOLE, 0}].push_back({{job, RT_WHOLE, 0, 1}, c});
             continue;
         }
-        // staging adds about one pass of LDS traffic to every task
-        const double ccol = (double)m1 * (std::log2((double)m1) + 2.0);
-        const double crow = (double)m2 * (std::log2((double)m2) + 2.0) * (blue ? 2.0 : 1.0);
+        // Power-of-two splits (every Bluestein job) run on the compile-time
+        // cores, which take exactly RAG_PTS/L sub-transforms per task.
+        const bool p2 = p2_split(m1, m2, j.g_col, j.g_row);
+        if (blue && !p2) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no power-of-two split for Bluestein length");
+        const int lc = blue ? RL_COLS_BLUE : p2 ? RL_COLS_P2 : RL_COLS;
+        const int lr = blue ? RL_ROWS_BLUE : p2 ? RL_ROWS_P2 : RL_ROWS;
+        // the generic passes stage through LDS: about one pass more traffic
+        const double extra = p2 ? 1.0 : 2.0;
+        const double ccol = (double)m1 * (std::log2((double)m1) + extra);
+        const double crow = (double)m2 * (std::log2((double)m2) + extra) * (blue ? 2.0 : 1.0);
+        const int l1 = p2 ? (int)m1 : 0, l2 = p2 ? (int)m2 : 0;
         for (int64_t c0 = 0; c0 < m2; c0 += j.g_col) {
             const int cnt = (int)std::min<int64_t>(j.g_col, m2 - c0);
-            ph[blue ? RL_COLS_BLUE : RL_COLS].push_back({{job, RT_COLS, (int)c0, cnt}, ccol * cnt});
-            if (blue) ph[RL_COLS2].push_back({{job, RT_COLS2, (int)c0, cnt}, ccol * cnt});
+            ph[{lc, l1}].push_back({{job, RT_COLS, (int)c0, cnt}, ccol * cnt});
+            if (blue) ph[{RL_COLS2, l1}].push_back({{job, RT_COLS2, (int)c0, cnt}, ccol * cnt});
         }
         for (int64_t r0 = 0; r0 < m1; r0 += j.g_row) {
             const int cnt = (int)std::min<int64_t>(j.g_row, m1 - r0);
-            ph[blue ? RL_ROWS_BLUE : RL_ROWS].push_back({{job, RT_ROWS, (int)r0, cnt}, crow * cnt});
+            ph[{lr, l2}].push_back({{job, RT_ROWS, (int)r0, cnt}, crow * cnt});
         }
     }
     r.total = off;
     // longest-processing-time order within each phase
     std::vector<RTask> tasks;
-    for (int p = 0; p < RAG_PHASES; ++p) {
-        std::stable_sort(ph[p].begin(), ph[p].end(), [](const Pending& a, const Pending& b) { return a.cost > b.cost; });
-        r.task_off[p] = (int)tasks.size();
-        r.ntask[p] = (int)ph[p].size();
-        for (auto& q : ph[p]) tasks.push_back(q.t);
+    r.launches.clear();
+    for (auto& kv : ph) {
+        auto& q = kv.second;
+        std::stable_sort(q.begin(), q.end(), [](const Pending& a, const Pending& b) { return a.cost > b.cost; });
+        RagLaunch l;
+        l.kind = kv.first.first;
+        l.L = kv.first.second;
+        l.task_off = (int)tasks.size();
+        l.ntask = (int)q.size();
+        for (auto& t : q) tasks.push_back(t.t);
+        r.launches.push_back(l);
     }
+    if ((int)r.launches.size() > RAG_COUNTERS) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "too many ragged launches");
     r.njobs = (int)jobs.size();
     AFFT_HIP(hipMalloc(&r.jobs_dev, std::max<size_t>(1, jobs.size()) * sizeof(RJob)));
     AFFT_HIP(hipMemcpy(r.jobs_dev, jobs.data(), jobs.size() * sizeof(RJob), hipMemcpyHostToDevice));
     AFFT_HIP(hipMalloc(&r.tasks_dev, std::max<size_t>(1, tasks.size()) * sizeof(RTask)));
     if (!tasks.empty())
         AFFT_HIP(hipMemcpy(r.tasks_dev, tasks.data(), tasks.size() * sizeof(RTask), hipMemcpyHostToDevice));
-    AFFT_HIP(hipMalloc(&r.counter, 64));
+    AFFT_HIP(hipMalloc(&r.counter, RAG_COUNTERS * sizeof(int)));
     for (int inv = 0; inv < 2; ++inv)
-        for (int p = 0; p < RAG_PHASES; ++p) {
+        for (RagLaunch& l : r.launches) {
+            const void* fn = rag_kernel(r.prec, l, inv ? 1 : -1);
+            if (!fn) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no ragged kernel for a task type");
             int per_cu = 0;
-            AFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rag_kernel(r.prec, p, inv ? 1 : -1),
-                                                                  RAG_THREADS, 0));
-            r.grid[inv][p] = std::max(1, per_cu) * std::max(1, r.ctx->props.multiProcessorCount);
+            AFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, RAG_THREADS, 0));
+            l.grid[inv] = std::max(1, per_cu) * std::max(1, r.ctx->props.multiProcessorCount);
         }
     AFFT_HIP(hipMalloc(&r.scratch, (size_t)std::max<int64_t>(1, r.scratch_pts) * es));
     return AFFT_OK;
 }
 
+// Enqueues the batch on s: the counter reset, then each chain of launches on
+// its own branch (branch 0 on s itself), joined back into s.  Works under
+// stream capture as well, where the events become graph edges.
 static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, hipStream_t s) {
     void* rargs[] = {&r.counter};
-    AFFT_HIP(hipLaunchKernel((const void*)&k_rag_reset, dim3(1), dim3(64), rargs, 0, s));
-    for (int p = 0; p < RAG_PHASES; ++p) {
-        if (r.ntask[p] == 0) continue;
+    AFFT_HIP(hipLaunchKernel((const void*)&k_rag_reset, dim3(1), dim3(RAG_COUNTERS), rargs, 0, s));
+    bool used[RAG_BRANCHES] = {};
+    for (const RagLaunch& l : r.launches) used[rag_branch(l.kind)] = true;
+    AFFT_HIP(hipEventRecord(r.fork, s));
+    for (int b = 1; b < RAG_BRANCHES; ++b)
+        if (used[b]) AFFT_HIP(hipStreamWaitEvent(r.branch[b - 1], r.fork, 0));
+    for (size_t p = 0; p < r.launches.size(); ++p) {
+        const RagLaunch& l = r.launches[p];
+        const int b = rag_branch(l.kind);
+        hipStream_t bs = b ? r.branch[b - 1] : s;
         RagArgs a{};
         a.in = src;
         a.out = dst;
         a.jobs = r.jobs_dev;
-        a.tasks = r.tasks_dev + r.task_off[p];
-        a.ntasks = r.ntask[p];
+        a.tasks = r.tasks_dev + l.task_off;
+        a.ntasks = l.ntask;
         a.dir = dir;
         a.counter = r.counter + p;
         a.scratch = r.scratch;
         void* args[] = {&a};
-        const int grid = std::min(r.grid[dir > 0 ? 1 : 0][p], r.ntask[p]);
-        AFFT_HIP(hipLaunchKernel(rag_kernel(r.prec, p, dir), dim3(grid), dim3(RAG_THREADS), args, 0, s));
+        const int grid = std::min(l.grid[dir > 0 ? 1 : 0], l.ntask);
+        AFFT_HIP(hipLaunchKernel(rag_kernel(r.prec, l, dir), dim3(grid), dim3(RAG_THREADS), args, 0, bs));
     }
+    for (int b = 1; b < RAG_BRANCHES; ++b)
+        if (used[b]) {
+            AFFT_HIP(hipEventRecord(r.join[b - 1], r.branch[b - 1]));
+            AFFT_HIP(hipStreamWaitEvent(s, r.join[b - 1], 0));
+        }
     return AFFT_OK;
 }
 
@@ -309,7 +421,13 @@ int afft_ragged_create(afft_context ctx, int precision, const int64_t* sizes, in
     int rc = ragged_build(*r);
     if (rc == AFFT_OK) {
         hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) rc = set_error(AFFT_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        for (int b = 0; b < RAG_BRANCHES - 1 && e == hipSuccess; ++b) {
+            e = hipStreamCreateWithFlags(&r->branch[b], hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&r->join[b], hipEventDisableTiming);
+        }
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&r->fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&r->done, hipEventDisableTiming);
+        if (e != hipSuccess) rc = set_error(AFFT_ERR_DEVICE, std::string("ragged streams: ") + hipGetErrorString(e));
     }
     if (rc != AFFT_OK) {
         std::string keep = afft_last_error();
@@ -340,7 +458,11 @@ int afft_ragged_exec(afft_ragged r, int direction, void* dst, const void* src, v
     if (dst == src) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "ragged batches run out of place");
     std::lock_guard<std::mutex> l(r->mu);
     AFFT_HIP(hipSetDevice(r->ctx->device));
-    int rc = ragged_launch(*r, direction, dst, src, stream ? (hipStream_t)stream : r->stream);
+    hipStream_t s = stream ? (hipStream_t)stream : r->stream;
+    bool track = false;
+    int rc = ragged_order_begin(*r, s, &track);
+    if (rc == AFFT_OK) rc = ragged_launch(*r, direction, dst, src, s);
+    if (rc == AFFT_OK) rc = ragged_order_end(*r, s, track);
     if (rc == AFFT_OK) clear_error();
     return rc;
 }
@@ -370,7 +492,11 @@ int afft_ragged_exec_graph(afft_ragged r, int direction, void* dst, const void* 
         if (e != hipSuccess) return set_error(AFFT_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         it = r->graphs.emplace(key, ex).first;
     }
+    bool track = false;
+    int rc = ragged_order_begin(*r, s, &track);
+    if (rc) return rc;
     AFFT_HIP(hipGraphLaunch(it->second, s));
+    if ((rc = ragged_order_end(*r, s, track))) return rc;
     clear_error();
     return AFFT_OK;
 }

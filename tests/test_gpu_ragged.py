@@ -77,3 +77,24 @@ def test_graph_replays_with_eager_work_between(ctx):
             torch.cuda.synchronize()
             y.zero_()
             del junk
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_ragged_power_of_two_split_lengths(ctx, dtype):
+    """Split jobs at both ends of the power-of-two core lengths: 8192 = 64 x
+    128 up to 2^20 = 1024 x 1024, and Bluestein paddings from 8192 (n = 2053)
+    to 2^21 = 1024 x 2048 (n = 1048573), beside generic smooth splits."""
+    sizes = [1 << 20, 1048573, 8192, 2053, 4099, 1 << 19, 524287, 16384, 100000, 12000]
+    prec = A.COMPLEX64 if dtype == np.complex64 else A.COMPLEX128
+    with A.Ragged(ctx, sizes, prec) as rg:
+        offs = rg.offsets()
+        x = synth.uniform_complex(13, (len(rg),), dtype)
+        xt = torch.from_numpy(x).to("cuda:0")
+        s = torch.cuda.Stream()
+        for direction, inverse in ((A.FORWARD, False), (A.INVERSE, True)):
+            yt = torch.zeros_like(xt)
+            with torch.cuda.stream(s):
+                rg.exec_graph(direction, yt.data_ptr(), xt.data_ptr(), s.cuda_stream)
+            s.synchronize()
+            print("inverse" if inverse else "forward", "worst err/tol",
+                  _check(sizes, offs, x, yt.cpu().numpy(), dtype, inverse=inverse))
