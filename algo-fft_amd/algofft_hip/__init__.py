@@ -345,14 +345,22 @@ class Ragged:
         self.close()
 
 
+CONV_LINEAR, CONV_CORRELATE, CONV_REAL = 0, 1, 2
+
+
 def convolve(ctx, a, b, correlate=False):
     """Convolve / CrossCorrelate (convolve.go:9-59, correlate.go:13-33) of two
-    host arrays on the device; returns len(a)+len(b)-1 values."""
+    complex host arrays, or ConvolveReal (convolve_real.go:9-80) of two real
+    ones, on the device; returns len(a)+len(b)-1 values."""
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b, dtype=a.dtype)
+    real = a.dtype in (np.float32, np.float64)
+    if real and correlate:
+        raise ValueError("correlation is defined for complex sequences")
     prec = _np_prec(a.dtype)
+    mode = CONV_REAL if real else (CONV_CORRELATE if correlate else CONV_LINEAR)
     out = np.empty(a.size + b.size - 1, dtype=a.dtype)
-    check(lib().afft_convolve_host(ctx.h, prec, int(correlate), out.ctypes.data_as(ctypes.c_void_p), out.size,
+    check(lib().afft_convolve_host(ctx.h, prec, mode, out.ctypes.data_as(ctypes.c_void_p), out.size,
                                    a.ctypes.data_as(ctypes.c_void_p), a.size, b.ctypes.data_as(ctypes.c_void_p),
                                    b.size))
     return out

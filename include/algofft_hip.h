@@ -277,20 +277,26 @@ int afft_mlp_destroy(afft_mlp m);
 int afft_context_set_cost_model(afft_context ctx, afft_mlp m);
 
 /* ---- convolution and correlation -------------------------------------------
- * Convolve / Convolve128 (convolve.go:9-115) and CrossCorrelate /
- * AutoCorrelate (correlate.go:6-79) on the device.  mode 0: linear
- * convolution; mode 1: cross-correlation (b reversed and conjugated; output
- * index k is lag k-(nb-1)).  Output length na+nb-1. */
+ * Convolve / Convolve128 (convolve.go:9-115), ConvolveReal
+ * (convolve_real.go:9-80) and CrossCorrelate / AutoCorrelate
+ * (correlate.go:6-79) on the device.  Output length na+nb-1.
+ *   AFFT_CONV_LINEAR     complex linear convolution
+ *   AFFT_CONV_CORRELATE  complex cross-correlation (b reversed and
+ *                        conjugated; output index k is lag k-(nb-1))
+ *   AFFT_CONV_REAL       real linear convolution: a, b and dst are float
+ *                        (AFFT_COMPLEX64) or double (AFFT_COMPLEX128) */
+enum { AFFT_CONV_LINEAR = 0, AFFT_CONV_CORRELATE = 1, AFFT_CONV_REAL = 2 };
 typedef struct afft_conv_s* afft_conv;
 int afft_conv_create(afft_context ctx, int precision, int64_t na, int64_t nb, int64_t batch, int mode,
                      afft_conv* out);
 /* Device pointers: a = batch rows of na, b = batch rows of nb, dst = batch
- * rows of na+nb-1 (complex elements); enqueued on stream (NULL = own). */
+ * rows of na+nb-1 (complex elements, real for AFFT_CONV_REAL); enqueued on
+ * stream (NULL = own). */
 int afft_conv_exec(afft_conv c, void* dst, const void* a, const void* b, void* stream);
 int afft_conv_destroy(afft_conv c);
 /* Host slices, one convolution, synchronised; the checks and errors of
- * Convolve (nil -> ErrNilSlice, empty -> ErrInvalidLength, dst length !=
- * na+nb-1 -> ErrLengthMismatch). */
+ * Convolve and ConvolveReal (nil -> ErrNilSlice, empty -> ErrInvalidLength,
+ * dst length != na+nb-1 -> ErrLengthMismatch). */
 int afft_convolve_host(afft_context ctx, int precision, int mode, void* dst, int64_t dst_len, const void* a,
                        int64_t na, const void* b, int64_t nb);
 
