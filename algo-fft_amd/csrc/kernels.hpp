@@ -124,11 +124,26 @@ __global__ void __launch_bounds__((KCfg<T, S, E, TPB, VAR>::THREADS)) k_fft(KArg
 
     if constexpr (VAR == VAR_4A) {
         const long long col = (a.tw4_by_y ? (long long)blockIdx.y : cg) + a.tw4_c0;
+        // W_L^{k*col} for k = t + TT*e (k < N and col < L/N, so k*col < L
+        // needs no reduction).  Lanes differ in col, so every table load
+        // gathers from scattered lines: look up every fourth e and step
+        // between them by W_L^{TT*col}, at most three products from a
+        // table value (10 table loads per thread instead of 2E).
+        if constexpr (E % 4 == 0) {
+            const cx<T> st = tw4<T>(a, (long long)TT * col);
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            // k < N and col < L/N, so k*col < L needs no reduction
-            const long long m = (long long)(t + TT * e) * col;
-            v[e] = twmul<DIR>(v[e], tw4<T>(a, m));
+            for (int e0 = 0; e0 < E; e0 += 4) {
+                cx<T> w = tw4<T>(a, (long long)(t + TT * e0) * col);
+                v[e0] = twmul<DIR>(v[e0], w);
+#pragma unroll
+                for (int e = e0 + 1; e < e0 + 4; ++e) {
+                    w = cmul(w, st);
+                    v[e] = twmul<DIR>(v[e], w);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = twmul<DIR>(v[e], tw4<T>(a, (long long)(t + TT * e) * col));
         }
     }
     if (a.scale != 1.0) {

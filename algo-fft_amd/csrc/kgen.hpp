@@ -45,8 +45,9 @@ __device__ __forceinline__ void dft_rt(int R, cx<T>* v) {
 
 // One runtime Stockham pass with a compile-time radix R, so the butterfly
 // stays in registers: thread t of T handles butterflies j = t, t+T, ... of
-// the N/R in the pass; load(n) gives input point n, store(k, v) takes output
-// point k = (j/Ns)*Ns*R + (j mod Ns) + r*Ns.
+// the N/R in the pass; load(n) gives input point n, store(k, v, r) takes
+// output point k = (j/Ns)*Ns*R + (j mod Ns) + r*Ns (r counts 0 .. R-1 within
+// the butterfly).
 template <int R, int DIR, typename T, class Load, class Store>
 __device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, unsigned NSM, const cx<T>* __restrict__ tw,
                                         Load load, Store store) {
@@ -64,7 +65,7 @@ __device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, unsigned N
         dft<R, DIR, 1>(v);
         const int base = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) store(base + r * NS, v[r]);
+        for (int r = 0; r < R; ++r) store(base + r * NS, v[r], r);
     }
 }
 
@@ -122,11 +123,18 @@ __global__ void __launch_bounds__(GEN_MAXTHREADS) k_gen(GArgs g) {
         // so no store or load has to choose its address space at run time
         auto load_g = [&](int n) -> cx<T> { return act ? src_g[(long long)n * ind] : mk<T>(0, 0); };
         auto load_s = [&](int n) -> cx<T> { return sm[lidx(rb, n)]; };
-        auto store_s = [&](int k, cx<T> y) { sm[lidx(wb, k)] = y; };
-        auto store_o = [&](int k, cx<T> y) {
+        auto store_s = [&](int k, cx<T> y, int) { sm[lidx(wb, k)] = y; };
+        // VAR_4A: W_L^{k*col} for the outputs k = base + r*Ns of a butterfly:
+        // a table value every fourth r, the step W_L^{Ns*col} in between
+        // (lanes differ in col, so the table gathers hit scattered lines)
+        const long long col = cg + a.tw4_c0;
+        cx<T> tw_step = mk<T>(1, 0), tw_cur = mk<T>(1, 0);
+        if constexpr (VAR == VAR_4A)
+            if (last) tw_step = tw4<T>(a, (long long)NS * col);
+        auto store_o = [&](int k, cx<T> y, int r) {
             if constexpr (VAR == VAR_4A) {
-                const long long m = (long long)k * (cg + a.tw4_c0);
-                y = twmul<DIR>(y, tw4<T>(a, m));
+                tw_cur = (r % 4 == 0) ? tw4<T>(a, (long long)k * col) : cmul(tw_cur, tw_step);
+                y = twmul<DIR>(y, tw_cur);
             }
             if (a.scale != 1.0) y = scl(y, (T)a.scale);
             if constexpr (VAR == VAR_ROW) {

@@ -452,10 +452,17 @@ __device__ __forceinline__ void p2_cols(const RagArgs a, const RJob& jb, const R
     }
     core<T, S, E, SD>(v, lds, t, (const cx<T>*)jb.s[0].tw, LdsCol<G>(c));
     cx<T>* __restrict__ sp = (cx<T>*)a.scratch + jb.scr_off + col;
+    // W_m^{k1*col} for k1 = t + TT*e: a table value every fourth e, the step
+    // W_m^{TT*col} in between (lanes differ in col: scattered table lines)
+    const cx<T> st = rag_tw<T>(jb, col * TT);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int k1 = t + TT * e;
-        sp[m2 * k1] = twmul<SD>(v[e], rag_tw<T>(jb, col * k1));
+    for (int e0 = 0; e0 < E; e0 += 4) {
+        cx<T> w = rag_tw<T>(jb, col * (t + TT * e0));
+#pragma unroll
+        for (int e = e0; e < e0 + 4; ++e) {
+            if (e > e0) w = cmul(w, st);
+            sp[m2 * (t + TT * e)] = twmul<SD>(v[e], w);
+        }
     }
 }
 

@@ -230,14 +230,25 @@ __global__ void __launch_bounds__((MfCfg<N, G>::T)) k_mfma(KArgs a) {
     }
     mf_passes<N, G, VAR, DIR>(a, lds, Ah, Al, in, out, c0, std::make_integer_sequence<int, C::P>{});
     if constexpr (VAR == VAR_4A || VAR == VAR_4B) {
-        const float s = (float)a.scale;
-        for (int i = tid; i < G * N; i += C::T) {
-            const int k = i / G, c = i - k * G;
-            const long long cg = c0 + c;
+        // thread tid stores points k = tid/G + s*(T/G) of column c = tid mod G
+        constexpr int STEPS = G * N / C::T, KS = C::T / G;
+        static_assert(C::T % G == 0 && (G * N) % C::T == 0, "whole columns per step");
+        const float sc = (float)a.scale;
+        const int c = tid % G, k0 = tid / G;
+        const long long cg = c0 + c;
+        const long long col = cg + a.tw4_c0;
+        // W_L^{k*col}: a table value every fourth step, the step factor
+        // W_L^{KS*col} in between (the table gathers are scattered lines)
+        cx<float> st = mk<float>(1, 0), w = mk<float>(1, 0);
+        if constexpr (VAR == VAR_4A) st = tw4<float>(a, (long long)KS * col);
+#pragma unroll
+        for (int s = 0; s < STEPS; ++s) {
+            const int k = k0 + s * KS;
+            if constexpr (VAR == VAR_4A) w = (s % 4 == 0) ? tw4<float>(a, (long long)k * col) : cmul(w, st);
             if (cg < a.count) {
                 cx<float> v = lds[c * C::PITCH + mf_pad(k)];
-                if constexpr (VAR == VAR_4A) v = twmul<DIR>(v, tw4<float>(a, (long long)k * (cg + a.tw4_c0)));
-                st_out(out + cg * a.out_cd + (long long)k * a.out_nd, scl(v, s), a.nt);
+                if constexpr (VAR == VAR_4A) v = twmul<DIR>(v, w);
+                st_out(out + cg * a.out_cd + (long long)k * a.out_nd, scl(v, sc), a.nt);
             }
         }
     }
