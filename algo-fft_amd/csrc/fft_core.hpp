@@ -87,21 +87,22 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
         for (int b = 0; b < B; ++b) {
             const int j = t + TT * b;
             const int k = j % NS;
-            const cx<T>* __restrict__ w = tw + S::twoff(p) + k * (R - 1);
+            // r-major table: w^r of butterfly k at (r-1)*NS + k
+            const cx<T>* __restrict__ w = tw + S::twoff(p) + k;
             if constexpr (R == 16 || R == 8) {
                 // Load w^1, w^2, w^4 (, w^8) and form the other powers as at
                 // most three products: a quarter of the table traffic, at a
                 // cost of a few ulps on the composite powers.
                 cx<T> ww[R];
                 ww[1] = w[0];
-                ww[2] = w[1];
-                ww[4] = w[3];
+                ww[2] = w[NS];
+                ww[4] = w[3 * NS];
                 ww[3] = cmul(ww[1], ww[2]);
                 ww[5] = cmul(ww[1], ww[4]);
                 ww[6] = cmul(ww[2], ww[4]);
                 ww[7] = cmul(ww[3], ww[4]);
                 if constexpr (R == 16) {
-                    ww[8] = w[7];
+                    ww[8] = w[7 * NS];
 #pragma unroll
                     for (int r = 9; r < 16; ++r) ww[r] = cmul(ww[r - 8], ww[8]);
                 }
@@ -109,7 +110,7 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
                 for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], ww[r]);
             } else {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], w[r - 1]);
+                for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], w[(r - 1) * NS]);
             }
         }
     }

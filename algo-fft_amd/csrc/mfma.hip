@@ -136,7 +136,7 @@ __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, const ha
                 if constexpr (GIN) x[i] = act ? in[cg * a.in_cd + n] : mk<float>(0, 0);
                 else x[i] = lds[c * C::PITCH + mf_pad(n)];
                 if constexpr (NS > 1) {
-                    if (r > 0) x[i] = twmul<DIR>(x[i], tw[TWO + (jb % NS) * 15 + r - 1]);
+                    if (r > 0) x[i] = twmul<DIR>(x[i], tw[TWO + (r - 1) * NS + jb % NS]);   // r-major table
                 }
             }
             mf_dft16(Ah, Al, x, y[it]);

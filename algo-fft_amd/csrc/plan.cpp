@@ -49,16 +49,18 @@ int upload_table(Plan& p, const void* host, size_t bytes, const void** dev) {
     return AFFT_OK;
 }
 
-// Twiddle table of a compile-time schedule: for pass i >= 1 with
+// Twiddle table of a radix schedule: for pass i >= 1 with
 // Ns = rad[0]*...*rad[i-1] and radix R: entries W_{Ns*R}^{k*r}, k < Ns,
-// 1 <= r < R, stored k-major.  Generated in FP64 and rounded once.
+// 1 <= r < R, stored r-major (entry (r-1)*Ns + k), so the lanes of a pass,
+// which hold consecutive k, load consecutive entries.  Generated in FP64 and
+// rounded once.
 static int schedule_twiddles(Plan& p, const KernelInfo* k, const void** dev) {
     std::vector<double> tw;
     int64_t ns = k->rad[0];
     for (int i = 1; i < k->nrad; ++i) {
         const int R = k->rad[i];
-        for (int64_t kk = 0; kk < ns; ++kk)
-            for (int r = 1; r < R; ++r) {
+        for (int r = 1; r < R; ++r)
+            for (int64_t kk = 0; kk < ns; ++kk) {
                 double re, im;
                 root_of_unity(kk * r, ns * R, &re, &im);
                 tw.push_back(re);
@@ -482,8 +484,8 @@ static int gen_args(Plan& p, int64_t n, bool rows, GArgs& g, int* tpt, int* lds,
         g.ns[i] = (int)ns;
         g.nsm[i] = mod_magic((unsigned)ns);
         g.twoff[i] = (int)(tw.size() / 2);
-        for (int64_t kk = 0; kk < ns; ++kk)
-            for (int r = 1; r < R; ++r) {
+        for (int r = 1; r < R; ++r)              // r-major, as schedule_twiddles
+            for (int64_t kk = 0; kk < ns; ++kk) {
                 double re, im;
                 root_of_unity(kk * r, ns * R, &re, &im);
                 tw.push_back(re);

@@ -159,8 +159,8 @@ static int schedule_into(Plan& tab, int prec, int64_t m, RSched& j, std::map<int
         j.nsm[i] = mod_magic((unsigned)ns);
         j.twoff[i] = off;
         if (i > 0) {
-            for (int64_t k = 0; k < ns; ++k)
-                for (int r = 1; r < rad[i]; ++r) {
+            for (int r = 1; r < rad[i]; ++r)     // r-major: entry (r-1)*ns + k
+                for (int64_t k = 0; k < ns; ++k) {
                     double re, im;
                     root_of_unity(k * r, ns * rad[i], &re, &im);
                     tw.push_back(re);
