@@ -133,9 +133,9 @@ __device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, in
                 for (int r = 0; r < R; ++r) v[b][r] = load(j + r * NR);
                 const int k = fast_mod(j, NS, NSM);
                 if (NS > 1) {
-                    const cx<T>* w = tw + k;   // r-major table: (r-1)*NS + k
+                    const cx<T>* w = tw + k * (R - 1);   // k-major table: k*(R-1) + r-1
 #pragma unroll
-                    for (int r = 1; r < R; ++r) v[b][r] = twmul<DIR>(v[b][r], w[(r - 1) * NS]);
+                    for (int r = 1; r < R; ++r) v[b][r] = twmul<DIR>(v[b][r], w[r - 1]);
                 }
                 dft<R, DIR, 1>(v[b]);
             }

@@ -484,8 +484,10 @@ static int gen_args(Plan& p, int64_t n, bool rows, GArgs& g, int* tpt, int* lds,
         g.ns[i] = (int)ns;
         g.nsm[i] = mod_magic((unsigned)ns);
         g.twoff[i] = (int)(tw.size() / 2);
-        for (int r = 1; r < R; ++r)              // r-major, as schedule_twiddles
-            for (int64_t kk = 0; kk < ns; ++kk) {
+        // k-major (entry k*(R-1) + r-1): the runtime passes read a butterfly's
+        // R-1 twiddles at compile-time offsets from one address
+        for (int64_t kk = 0; kk < ns; ++kk)
+            for (int r = 1; r < R; ++r) {
                 double re, im;
                 root_of_unity(kk * r, ns * R, &re, &im);
                 tw.push_back(re);

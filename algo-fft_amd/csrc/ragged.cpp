@@ -130,9 +130,14 @@ static int ragged_order_end(afft_ragged_s& r, hipStream_t s, bool track) {
 }
 
 // Radix schedule and per-pass twiddles of length m (m <= RAG_WHOLE), shared
-// between jobs through `cache`.
-static int schedule_into(Plan& tab, int prec, int64_t m, RSched& j, std::map<int64_t, RSched>& cache) {
-    auto it = cache.find(m);
+// between jobs through `cache`.  The table is r-major (entry (r-1)*Ns + k,
+// the layout of the compile-time cores) for the power-of-two split tasks and
+// k-major (entry k*(R-1) + r-1, one base address per butterfly) for the
+// runtime passes.
+using SchedKey = std::pair<int64_t, bool>;
+static int schedule_into(Plan& tab, int prec, int64_t m, bool rmajor, RSched& j, std::map<SchedKey, RSched>& cache) {
+    const SchedKey key{m, rmajor};
+    auto it = cache.find(key);
     if (it != cache.end()) {
         j = it->second;
         return AFFT_OK;
@@ -143,7 +148,7 @@ static int schedule_into(Plan& tab, int prec, int64_t m, RSched& j, std::map<int
         j.rad[0] = 1;
         j.ns[0] = 1;
         j.nsm[0] = mod_magic(1);
-        cache[m] = j;
+        cache[key] = j;
         return AFFT_OK;
     }
     std::vector<int> rad = smooth_schedule(m);
@@ -159,14 +164,16 @@ static int schedule_into(Plan& tab, int prec, int64_t m, RSched& j, std::map<int
         j.nsm[i] = mod_magic((unsigned)ns);
         j.twoff[i] = off;
         if (i > 0) {
-            for (int r = 1; r < rad[i]; ++r)     // r-major: entry (r-1)*ns + k
-                for (int64_t k = 0; k < ns; ++k) {
-                    double re, im;
-                    root_of_unity(k * r, ns * rad[i], &re, &im);
-                    tw.push_back(re);
-                    tw.push_back(im);
-                }
-            off += (int)(ns * (rad[i] - 1));
+            const int R = rad[i];
+            for (int64_t e = 0; e < ns * (R - 1); ++e) {
+                const int64_t k = rmajor ? e % ns : e / (R - 1);
+                const int r = rmajor ? (int)(e / ns) + 1 : (int)(e % (R - 1)) + 1;
+                double re, im;
+                root_of_unity(k * r, ns * R, &re, &im);
+                tw.push_back(re);
+                tw.push_back(im);
+            }
+            off += (int)(ns * (R - 1));
         }
         ns *= rad[i];
     }
@@ -179,7 +186,7 @@ static int schedule_into(Plan& tab, int prec, int64_t m, RSched& j, std::map<int
     } else {
         if ((rc = upload_table(tab, tw.data(), tw.size() * 8, &j.tw))) return rc;
     }
-    cache[m] = j;
+    cache[key] = j;
     return AFFT_OK;
 }
 
@@ -233,7 +240,7 @@ static bool p2_split(int64_t m1, int64_t m2, int g_col, int g_row) {
 static int ragged_build(afft_ragged_s& r) {
     const int es = r.prec ? 16 : 8;
     std::vector<RJob> jobs(r.sizes.size());
-    std::map<int64_t, RSched> scache;
+    std::map<SchedKey, RSched> scache;
     std::map<std::pair<int64_t, int64_t>, BlueTables> bcache;
     struct TwSplit { int shift; const void* lo; const void* hi; };
     std::map<int64_t, TwSplit> tcache;
@@ -266,12 +273,17 @@ static int ragged_build(afft_ragged_s& r) {
         const bool whole = m <= RAG_WHOLE;
         int rc;
         if (whole) {
-            if ((rc = schedule_into(r.tables, r.prec, m, j.s[0], scache))) return rc;
+            if ((rc = schedule_into(r.tables, r.prec, m, false, j.s[0], scache))) return rc;
         } else {
             j.m1 = (int)m1;
             j.m2 = (int)m2;
-            if ((rc = schedule_into(r.tables, r.prec, m1, j.s[0], scache))) return rc;
-            if ((rc = schedule_into(r.tables, r.prec, m2, j.s[1], scache))) return rc;
+            // at most RAG_PTS points per task: every thread then holds at
+            // most 16/R butterflies of a radix-R pass
+            j.g_col = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m1, m2}));
+            j.g_row = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m2, m1}));
+            const bool rmaj = p2_split(m1, m2, j.g_col, j.g_row);
+            if ((rc = schedule_into(r.tables, r.prec, m1, rmaj, j.s[0], scache))) return rc;
+            if ((rc = schedule_into(r.tables, r.prec, m2, rmaj, j.s[1], scache))) return rc;
             auto tt = tcache.find(m);
             if (tt == tcache.end()) {
                 TwSplit ts{};
@@ -281,10 +293,6 @@ static int ragged_build(afft_ragged_s& r) {
             j.shift = tt->second.shift;
             j.twlo = tt->second.lo;
             j.twhi = tt->second.hi;
-            // at most RAG_PTS points per task: every thread then holds at
-            // most 16/R butterflies of a radix-R pass
-            j.g_col = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m1, m2}));
-            j.g_row = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m2, m1}));
             j.scr_off = r.scratch_pts;
             r.scratch_pts += m;
         }
