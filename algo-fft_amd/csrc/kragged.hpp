@@ -63,6 +63,7 @@ enum {
     RL_ROWS_P2,
     RL_ROWS_BLUE,
     RL_COLS2,
+    RL_WHOLE_P2,
     RL_WHOLE,
     RL_WHOLE_BLUE,
     RL_COUNT
@@ -104,6 +105,7 @@ struct RagArgs {
     int dir;
     int* counter;                // this launch's queue counter
     void* scratch;               // the split jobs' S arrays
+    const int* joblist;          // grouped whole jobs: job ids, tasks index it
 };
 
 template <typename T>
@@ -412,6 +414,8 @@ __device__ void rag_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>
 
 template <int L>
 struct P2Sched;
+template <> struct P2Sched<16> { using S = Sched<16>; };
+template <> struct P2Sched<32> { using S = Sched<2, 16>; };
 template <> struct P2Sched<64> { using S = Sched<4, 16>; };
 template <> struct P2Sched<128> { using S = Sched<8, 16>; };
 template <> struct P2Sched<256> { using S = Sched<16, 16>; };
@@ -525,6 +529,37 @@ __device__ __forceinline__ void p2_rows_blue(const RagArgs a, const RJob& jb, co
     }
 }
 
+// Whole power-of-two jobs of one length L: a task holds up to G = RAG_PTS/L
+// of them (their ids at a.joblist[first .. first+count)), L/16 threads each,
+// loaded straight into registers and stored in natural order.  The table
+// pointer is the first job's: every job of the task has the same schedule.
+template <typename T, int L>
+struct P2Whole {
+    static constexpr int PADSH = PadFor<T, L>::value;
+    static constexpr int LDS = RAG_PTS / L * LdsRow<L, PADSH>::SLAB;
+};
+
+template <typename T, int L, int DIR>
+__device__ __forceinline__ void p2_whole(const RagArgs a, const RTask& tk, cx<T>* lds) {
+    using S = typename P2Sched<L>::S;
+    constexpr int E = 16, TT = L / E;
+    const int tid = threadIdx.x, c = tid / TT, t = tid % TT;
+    const bool act = c < tk.count;
+    const int jid = a.joblist[tk.first + (act ? c : 0)];
+    const cx<T>* __restrict__ x = (const cx<T>*)a.in + a.jobs[jid].in_off;
+    const cx<T>* __restrict__ tw = (const cx<T>*)a.jobs[a.joblist[tk.first]].s[0].tw;
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = x[t + TT * e];
+    core<T, S, E, DIR>(v, lds, t, tw, LdsRow<L, P2Whole<T, L>::PADSH>(c));
+    if (act) {
+        const T sc = DIR > 0 ? T(1) / T(L) : T(1);
+        cx<T>* __restrict__ y = (cx<T>*)a.out + a.jobs[jid].out_off;
+#pragma unroll
+        for (int e = 0; e < E; ++e) y[t + TT * e] = scl(v[e], sc);
+    }
+}
+
 // Second column task of a Bluestein job: columns p0 .. p0+G-1 of S, inverse
 // FFT over k1, output z[p + m2*q] times the post-chirp (and 1/n for the
 // inverse transform), cropped to n.
@@ -601,6 +636,12 @@ template <typename T, int L>
 __global__ void __launch_bounds__(RAG_THREADS) k_rag_cols2(RagArgs a) {
     __shared__ cx<T> lds[RAG_PTS];
     rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_cols2<T, L>(a, jb, tk, lds); });
+}
+
+template <typename T, int L, int DIR>
+__global__ void __launch_bounds__(RAG_THREADS) k_rag_whole_p2(RagArgs a) {
+    __shared__ cx<T> lds[P2Whole<T, L>::LDS];
+    rag_queue(a, [=](const RJob&, const RTask& tk) { p2_whole<T, L, DIR>(a, tk, lds); });
 }
 
 template <typename T, int DIR>

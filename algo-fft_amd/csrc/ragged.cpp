@@ -72,6 +72,7 @@ struct afft_ragged_s {
     Plan tables;                  // owns the device tables
     RJob* jobs_dev = nullptr;
     RTask* tasks_dev = nullptr;
+    int* joblist_dev = nullptr;   // job ids of the grouped whole-job tasks
     int njobs = 0;
     std::vector<RagLaunch> launches;    // in launch order
     int* counter = nullptr;
@@ -100,6 +101,7 @@ afft_ragged_s::~afft_ragged_s() {
     for (auto& kv : per_size) afft_plan_destroy(kv.second);
     if (jobs_dev) (void)hipFree(jobs_dev);
     if (tasks_dev) (void)hipFree(tasks_dev);
+    if (joblist_dev) (void)hipFree(joblist_dev);
     if (counter) (void)hipFree(counter);
     if (scratch) (void)hipFree(scratch);
     if (stream) (void)hipStreamDestroy(stream);
@@ -241,6 +243,7 @@ static int ragged_build(afft_ragged_s& r) {
     const int es = r.prec ? 16 : 8;
     std::vector<RJob> jobs(r.sizes.size());
     std::map<SchedKey, RSched> scache;
+    std::map<int, std::vector<int>> groups;   // whole power-of-two jobs by length
     std::map<std::pair<int64_t, int64_t>, BlueTables> bcache;
     struct TwSplit { int shift; const void* lo; const void* hi; };
     std::map<int64_t, TwSplit> tcache;
@@ -272,8 +275,11 @@ static int ragged_build(afft_ragged_s& r) {
         j.blue = blue;
         const bool whole = m <= RAG_WHOLE;
         int rc;
+        // whole plain power-of-two jobs of 16 points or more run grouped on
+        // the compile-time cores (r-major table), the rest on runtime passes
+        const bool whole_p2 = whole && !blue && m >= 16 && (m & (m - 1)) == 0;
         if (whole) {
-            if ((rc = schedule_into(r.tables, r.prec, m, false, j.s[0], scache))) return rc;
+            if ((rc = schedule_into(r.tables, r.prec, m, whole_p2, j.s[0], scache))) return rc;
         } else {
             j.m1 = (int)m1;
             j.m2 = (int)m2;
@@ -311,6 +317,10 @@ static int ragged_build(afft_ragged_s& r) {
             j.filt[1] = it->second.filt[1];
         }
         const int job = (int)i;
+        if (whole_p2) {
+            groups[(int)m].push_back(job);
+            continue;
+        }
         if (whole) {
             const double c = (blue ? 2.0 : 1.0) * m * std::log2(std::max<double>(2.0, (double)m));
             ph[{blue ? RL_WHOLE_BLUE : RL_WHOLE, 0}].push_back({{job, RT_WHOLE, 0, 1}, c});
@@ -338,6 +348,17 @@ static int ragged_build(afft_ragged_s& r) {
         }
     }
     r.total = off;
+    // grouped whole jobs: RAG_PTS/L jobs of length L per task, ids in joblist
+    std::vector<int> joblist;
+    for (auto& g : groups) {
+        const int L = g.first, per = RAG_PTS / L;
+        for (size_t q = 0; q < g.second.size(); q += per) {
+            const int cnt = (int)std::min<size_t>(per, g.second.size() - q);
+            const int first = (int)joblist.size();
+            joblist.insert(joblist.end(), g.second.begin() + q, g.second.begin() + q + cnt);
+            ph[{RL_WHOLE_P2, L}].push_back({{g.second[q], RT_WHOLE, first, cnt}, (double)cnt});
+        }
+    }
     // longest-processing-time order within each phase
     std::vector<RTask> tasks;
     r.launches.clear();
@@ -356,6 +377,9 @@ static int ragged_build(afft_ragged_s& r) {
     r.njobs = (int)jobs.size();
     AFFT_HIP(hipMalloc(&r.jobs_dev, std::max<size_t>(1, jobs.size()) * sizeof(RJob)));
     AFFT_HIP(hipMemcpy(r.jobs_dev, jobs.data(), jobs.size() * sizeof(RJob), hipMemcpyHostToDevice));
+    AFFT_HIP(hipMalloc(&r.joblist_dev, std::max<size_t>(1, joblist.size()) * sizeof(int)));
+    if (!joblist.empty())
+        AFFT_HIP(hipMemcpy(r.joblist_dev, joblist.data(), joblist.size() * sizeof(int), hipMemcpyHostToDevice));
     AFFT_HIP(hipMalloc(&r.tasks_dev, std::max<size_t>(1, tasks.size()) * sizeof(RTask)));
     if (!tasks.empty())
         AFFT_HIP(hipMemcpy(r.tasks_dev, tasks.data(), tasks.size() * sizeof(RTask), hipMemcpyHostToDevice));
@@ -392,6 +416,7 @@ static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, 
         a.out = dst;
         a.jobs = r.jobs_dev;
         a.tasks = r.tasks_dev + l.task_off;
+        a.joblist = r.joblist_dev;
         a.ntasks = l.ntask;
         a.dir = dir;
         a.counter = r.counter + p;

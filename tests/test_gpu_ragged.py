@@ -98,3 +98,27 @@ def test_ragged_power_of_two_split_lengths(ctx, dtype):
             s.synchronize()
             print("inverse" if inverse else "forward", "worst err/tol",
                   _check(sizes, offs, x, yt.cpu().numpy(), dtype, inverse=inverse))
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_ragged_grouped_small_power_of_two_jobs(ctx, dtype):
+    """Whole power-of-two jobs of one length run RAG_PTS/L to a task: full and
+    partial groups of every length 16 .. 4096, interleaved with lengths that
+    stay on the runtime passes (2, 4, 8, smooth, prime)."""
+    rng = np.random.default_rng(21)
+    sizes = [16] * 300 + [32] * 130 + [64] * 65 + [128] * 33 + [256] * 17 + [512] * 9 + [1024] * 5 + \
+        [2048] * 3 + [4096] * 2 + [2, 4, 8, 3, 1000, 17, 4099]
+    sizes = [int(v) for v in rng.permutation(sizes)]
+    prec = A.COMPLEX64 if dtype == np.complex64 else A.COMPLEX128
+    with A.Ragged(ctx, sizes, prec) as rg:
+        offs = rg.offsets()
+        x = synth.uniform_complex(17, (len(rg),), dtype)
+        xt = torch.from_numpy(x).to("cuda:0")
+        s = torch.cuda.Stream()
+        for direction, inverse in ((A.FORWARD, False), (A.INVERSE, True)):
+            yt = torch.zeros_like(xt)
+            with torch.cuda.stream(s):
+                rg.exec(direction, yt.data_ptr(), xt.data_ptr(), s.cuda_stream)
+            s.synchronize()
+            print("inverse" if inverse else "forward", "worst err/tol",
+                  _check(sizes, offs, x, yt.cpu().numpy(), dtype, inverse=inverse))
