@@ -286,24 +286,36 @@ __device__ __forceinline__ void rag_stage_rows(const cx<T>* __restrict__ src, in
     __syncthreads();
 }
 
-// A whole job in one block: x staged into LDS with contiguous lanes, the
-// passes in place, the last pass writing y (its output index runs
-// contiguously across the threads as well).
+// Whole jobs of one length L on the runtime passes: a task holds up to
+// G = jb.g_col of them (ids at a.joblist[first .. first+count)), one LDS row
+// of pitch L + 1 and RAG_THREADS/G threads each.  The inputs are staged with
+// contiguous lanes along each row; the last pass writes each job's output.
 template <typename T, int DIR>
-__device__ void rag_whole(const RagArgs a, const RJob& jb, cx<T>* lds) {
-    const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
-    cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
-    const int m = jb.m, tid = threadIdx.x;
+__device__ void rag_whole(const RagArgs a, const RJob& j0, const RTask& tk, cx<T>* lds, long long* base) {
+    const int L = j0.m, P = L + 1, TPT = RAG_THREADS / j0.g_col, cnt = tk.count, tid = threadIdx.x;
+    const int total = cnt * L;
+    if (tid < cnt) base[tid] = a.jobs[a.joblist[tk.first + tid]].in_off;
+    __syncthreads();
+    const cx<T>* __restrict__ x = (const cx<T>*)a.in;
+    RowWalk rl(L), rw(L);
     rag_staged<sizeof(cx<T>)>(
-        flat_steps(m), [&](int s) { return x[min(tid + s * RAG_THREADS, m - 1)]; },
+        flat_steps(total),
+        [&](int) {
+            const cx<T> v = x[base[min(rl.c, cnt - 1)] + rl.n];
+            rl.next();
+            return v;
+        },
         [&](int s, cx<T> v) {
             const int i = tid + s * RAG_THREADS;
-            if (i < m) lds[i] = v;
+            if (i < total) lds[i + rw.c] = v;
+            rw.next();
         });
     __syncthreads();
-    const T sc = DIR > 0 ? T(1) / T(m) : T(1);
+    const int grp = tid / TPT;
+    cx<T>* __restrict__ y = (cx<T>*)a.out + (grp < cnt ? a.jobs[a.joblist[tk.first + grp]].out_off : 0);
+    const T sc = DIR > 0 ? T(1) / T(L) : T(1);
     auto sink = [&](int, int k, cx<T> v) { y[k] = scl(v, sc); };
-    rag_passes<T, DIR>(jb.s[0], m, m, 1, RAG_THREADS, lds, sink);
+    rag_passes<T, DIR>(j0.s[0], L, P, cnt, TPT, lds, sink);
 }
 
 // A whole Bluestein job: pre-chirp and zero padding on the staging load,
@@ -647,7 +659,8 @@ __global__ void __launch_bounds__(RAG_THREADS) k_rag_whole_p2(RagArgs a) {
 template <typename T, int DIR>
 __global__ void __launch_bounds__(RAG_THREADS) k_rag_whole(RagArgs a) {
     __shared__ cx<T> lds[RAG_BUF];
-    rag_queue(a, [=](const RJob& jb, const RTask&) { rag_whole<T, DIR>(a, jb, lds); });
+    __shared__ long long base[RAG_THREADS];
+    rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_whole<T, DIR>(a, jb, tk, lds, base); });
 }
 
 template <typename T>

@@ -244,6 +244,7 @@ static int ragged_build(afft_ragged_s& r) {
     std::vector<RJob> jobs(r.sizes.size());
     std::map<SchedKey, RSched> scache;
     std::map<int, std::vector<int>> groups;   // whole power-of-two jobs by length
+    std::map<int, std::vector<int>> gen_groups;   // other whole plain jobs by length
     std::map<std::pair<int64_t, int64_t>, BlueTables> bcache;
     struct TwSplit { int shift; const void* lo; const void* hi; };
     std::map<int64_t, TwSplit> tcache;
@@ -321,6 +322,12 @@ static int ragged_build(afft_ragged_s& r) {
             groups[(int)m].push_back(job);
             continue;
         }
+        if (whole && !blue) {
+            // RAG_PTS points per task at most, TPT = RAG_THREADS/G threads a job
+            j.g_col = pow2_floor(std::min<int64_t>(RAG_THREADS, RAG_PTS / m));
+            gen_groups[(int)m].push_back(job);
+            continue;
+        }
         if (whole) {
             const double c = (blue ? 2.0 : 1.0) * m * std::log2(std::max<double>(2.0, (double)m));
             ph[{blue ? RL_WHOLE_BLUE : RL_WHOLE, 0}].push_back({{job, RT_WHOLE, 0, 1}, c});
@@ -357,6 +364,16 @@ static int ragged_build(afft_ragged_s& r) {
             const int first = (int)joblist.size();
             joblist.insert(joblist.end(), g.second.begin() + q, g.second.begin() + q + cnt);
             ph[{RL_WHOLE_P2, L}].push_back({{g.second[q], RT_WHOLE, first, cnt}, (double)cnt});
+        }
+    }
+    for (auto& g : gen_groups) {
+        const int L = g.first, per = jobs[g.second[0]].g_col;
+        const double c = L * std::log2(std::max(2.0, (double)L));
+        for (size_t q = 0; q < g.second.size(); q += per) {
+            const int cnt = (int)std::min<size_t>(per, g.second.size() - q);
+            const int first = (int)joblist.size();
+            joblist.insert(joblist.end(), g.second.begin() + q, g.second.begin() + q + cnt);
+            ph[{RL_WHOLE, 0}].push_back({{g.second[q], RT_WHOLE, first, cnt}, c * cnt});
         }
     }
     // longest-processing-time order within each phase
