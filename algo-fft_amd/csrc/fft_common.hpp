@@ -21,12 +21,16 @@ struct __attribute__((aligned(2 * sizeof(T)))) cx {
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
 
-// Output store; nt marks it non-temporal (no allocation in the last-level
-// cache), for outputs too large to be re-read from there.
+// Non-temporal flags of a launch (KArgs::nt): streaming stores of the
+// output, and streaming loads of the input, for data too large to be re-read
+// from the last-level cache.
+constexpr int NT_STORE = 1, NT_LOAD = 2;
+
+// Output store; non-temporal under NT_STORE.
 template <typename T>
 __device__ __forceinline__ void st_out(cx<T>* p, cx<T> v, int nt) {
     typedef T v2 __attribute__((ext_vector_type(2)));
-    if (nt) {
+    if (nt & NT_STORE) {
         const v2 val = {v.x, v.y};
         __builtin_nontemporal_store(val, (v2*)p);
     } else {
@@ -43,6 +47,14 @@ __device__ __forceinline__ int fast_mod(int j, int d, unsigned m) {
     const unsigned q = __umulhi((unsigned)j, m);
     const int r = j - (int)(q * (unsigned)d);
     return r >= d ? r - d : r;
+}
+
+// Non-temporal input load.
+template <typename T>
+__device__ __forceinline__ cx<T> ld_nt(const cx<T>* p) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 d = __builtin_nontemporal_load((const v2*)p);
+    return mk<T>(d.x, d.y);
 }
 
 template <typename T>

@@ -42,7 +42,7 @@ struct KArgs {
     int fconj;               // use conj(fvec)
     const void* fvec;
     long long flen;
-    int nt;                  // non-temporal stores of the output
+    int nt;                  // NT_STORE / NT_LOAD: non-temporal output stores / input loads
 };
 
 enum { FUSE_NONE = 0, FUSE_CHIRP_IN = 1, FUSE_MUL_OUT = 2, FUSE_CROP_OUT = 3 };
@@ -95,7 +95,16 @@ __global__ void __launch_bounds__((KCfg<T, S, E, TPB, VAR>::THREADS)) k_fft(KArg
     const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
     cx<T> v[E];
 
-    if constexpr (VAR == VAR_ROW || VAR == VAR_4B) {
+    if constexpr (VAR == VAR_ROW) {
+        const cx<T>* src = in + cg * a.in_cd;
+        if (a.nt & NT_LOAD) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = act ? ld_nt(src + t + TT * e) : mk<T>(0, 0);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = act ? src[t + TT * e] : mk<T>(0, 0);
+        }
+    } else if constexpr (VAR == VAR_4B) {
         const cx<T>* src = in + cg * a.in_cd;
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = act ? src[t + TT * e] : mk<T>(0, 0);
