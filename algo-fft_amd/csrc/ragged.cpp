@@ -424,9 +424,14 @@ static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, 
     AFFT_HIP(hipEventRecord(r.fork, s));
     for (int b = 1; b < RAG_BRANCHES; ++b)
         if (used[b]) AFFT_HIP(hipStreamWaitEvent(r.branch[b - 1], r.fork, 0));
+    // Enqueue the longest chain first (Bluestein: three dependent passes),
+    // then the others; each branch keeps its own launch order.
+    static constexpr int order[RAG_BRANCHES] = {2, 1, 0, 3};
+    for (int ob = 0; ob < RAG_BRANCHES; ++ob)
     for (size_t p = 0; p < r.launches.size(); ++p) {
         const RagLaunch& l = r.launches[p];
         const int b = rag_branch(l.kind);
+        if (b != order[ob]) continue;
         hipStream_t bs = b ? r.branch[b - 1] : s;
         RagArgs a{};
         a.in = src;
