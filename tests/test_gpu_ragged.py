@@ -122,3 +122,29 @@ def test_ragged_grouped_small_power_of_two_jobs(ctx, dtype):
             s.synchronize()
             print("inverse" if inverse else "forward", "worst err/tol",
                   _check(sizes, offs, x, yt.cpu().numpy(), dtype, inverse=inverse))
+
+
+def test_ragged_executions_on_several_streams(ctx):
+    """Executions share the queue counters and the split jobs' scratch: one
+    enqueued on another stream than the previous one waits for it, so
+    back-to-back executions on four streams all give the single-stream bits."""
+    sizes = synth.ragged_sizes(23, 200, 16, 70000)
+    with A.Ragged(ctx, sizes) as rg:
+        xs = [torch.from_numpy(synth.uniform_complex(30 + i, (len(rg),), np.complex64)).to("cuda:0")
+              for i in range(4)]
+        refs = []
+        for x in xs:
+            y = torch.zeros_like(x)
+            rg.exec(A.FORWARD, y.data_ptr(), x.data_ptr())
+            torch.cuda.synchronize()
+            refs.append(y)
+        streams = [torch.cuda.Stream() for _ in range(4)]
+        outs = [torch.zeros_like(x) for x in xs]
+        for _ in range(2):
+            for s, x, y in zip(streams, xs, outs):
+                rg.exec(A.FORWARD, y.data_ptr(), x.data_ptr(), s.cuda_stream)
+            for s, x, y in zip(streams, xs, outs):
+                rg.exec_graph(A.FORWARD, y.data_ptr(), x.data_ptr(), s.cuda_stream)
+        torch.cuda.synchronize()
+        for y, r in zip(outs, refs):
+            assert torch.equal(y, r)
