@@ -559,5 +559,48 @@ inline Error NewPlanReal3D(int64_t d, int64_t h, int64_t w, std::unique_ptr<Desc
     return DescPlan::New(nd_desc(AFFT_R2C, PrecisionKind::Complex64, {d, h, w}), o, out);
 }
 
+// ---- convolution and correlation on the device --------------------------------
+// Convolve / Convolve128 (convolve.go:9-115), ConvolveReal (convolve_real.go:9-80),
+// CrossCorrelate / AutoCorrelate (correlate.go:6-79): the reference's checks in
+// its order (nil, empty, dst length), then one synchronised device call on a
+// context of `device`.
+namespace detail {
+inline Error convolve_on(int device, PrecisionKind p, int mode, void* dst, int64_t dl, const void* a, int64_t na,
+                         const void* b, int64_t nb) {
+    if (!dst || !a || !b) return ErrNilSlice;
+    if (na < 1 || nb < 1) return ErrInvalidLength;
+    if (dl != na + nb - 1) return ErrLengthMismatch;
+    afft_context ctx = nullptr;
+    Error e = from_status(afft_context_create(device, &ctx));
+    if (e) return e;
+    e = from_status(afft_convolve_host(ctx, (int)p, mode, dst, dl, a, na, b, nb));
+    afft_context_destroy(ctx);
+    return e;
+}
+}  // namespace detail
+
+template <typename T>
+inline Error Convolve(Slice<T> dst, Slice<const T> a, Slice<const T> b, int device = 0) {
+    return detail::convolve_on(device, precision_of<T>::value, AFFT_CONV_LINEAR, dst.data, dst.len, a.data, a.len,
+                               b.data, b.len);
+}
+template <typename T>
+inline Error CrossCorrelate(Slice<T> dst, Slice<const T> a, Slice<const T> b, int device = 0) {
+    return detail::convolve_on(device, precision_of<T>::value, AFFT_CONV_CORRELATE, dst.data, dst.len, a.data, a.len,
+                               b.data, b.len);
+}
+template <typename T>
+inline Error AutoCorrelate(Slice<T> dst, Slice<const T> a, int device = 0) {
+    return CrossCorrelate<T>(dst, a, a, device);
+}
+inline Error ConvolveReal(Slice<float> dst, Slice<const float> a, Slice<const float> b, int device = 0) {
+    return detail::convolve_on(device, PrecisionKind::Complex64, AFFT_CONV_REAL, dst.data, dst.len, a.data, a.len,
+                               b.data, b.len);
+}
+inline Error ConvolveReal64(Slice<double> dst, Slice<const double> a, Slice<const double> b, int device = 0) {
+    return detail::convolve_on(device, PrecisionKind::Complex128, AFFT_CONV_REAL, dst.data, dst.len, a.data, a.len,
+                               b.data, b.len);
+}
+
 }  // namespace gpu
 }  // namespace algofft

@@ -177,9 +177,34 @@ static void gpu_tests() {
     }
 }
 
+// Convolve / ConvolveReal / CrossCorrelate / AutoCorrelate through the mirror
+// (convolve_test.go:14 and convolve_real_test.go:10-29 known answers; checks
+// before any device call in the cpu mode).
+static void conv_tests(bool gpu) {
+    std::vector<c64> a{c64(1, 0), c64(2, 0), c64(3, 0)}, b{c64(4, 0), c64(5, 0)}, d(4), shortd(3);
+    CHECK(Convolve<c64>(Slice<c64>(), a, b).Is(ErrNilSlice), "nil dst");
+    CHECK(Convolve<c64>(d, Slice<const c64>(a.data(), 0), b).Is(ErrInvalidLength), "empty a");
+    CHECK(Convolve<c64>(shortd, a, b).Is(ErrLengthMismatch), "short dst");
+    std::vector<float> ra{1, 2, 3}, rb{4, 5}, rd(4);
+    CHECK(ConvolveReal(Slice<float>(), ra, rb).Is(ErrNilSlice), "real nil dst");
+    if (!gpu) return;
+    CHECK(!Convolve<c64>(d, a, b), "Convolve");
+    const float want[4] = {4, 13, 22, 15};
+    for (int i = 0; i < 4; ++i) CHECK(std::abs(d[i] - c64(want[i], 0)) < 1e-4f, "conv[%d]", i);
+    CHECK(!ConvolveReal(rd, ra, rb), "ConvolveReal");
+    for (int i = 0; i < 4; ++i) CHECK(std::fabs(rd[i] - want[i]) < 1e-4f, "real conv[%d]", i);
+    // AutoCorrelate of [1, i]: lags -1, 0, 1 -> [-i, 2, i]
+    std::vector<c128> x{c128(1, 0), c128(0, 1)}, ac(3);
+    CHECK(!AutoCorrelate<c128>(ac, x), "AutoCorrelate");
+    CHECK(std::abs(ac[0] - c128(0, -1)) < 1e-12 && std::abs(ac[1] - c128(2, 0)) < 1e-12 &&
+              std::abs(ac[2] - c128(0, 1)) < 1e-12,
+          "autocorrelation of [1, i]");
+}
+
 int main(int argc, char** argv) {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     cpu_tests();
+    conv_tests(gpu);
     if (gpu) gpu_tests();
     std::printf("%s tests: %s (%d failures)\n", gpu ? "gpu" : "cpu", failures ? "FAIL" : "ok", failures);
     return failures ? 1 : 0;
