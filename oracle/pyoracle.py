@@ -32,6 +32,7 @@ def lib():
             getattr(L, "or_fftnd_" + sfx).argtypes = [i, P, P, P, i]
             getattr(L, "or_rfftnd_" + sfx).argtypes = [i, P, P, P]
             getattr(L, "or_convolve_" + sfx).argtypes = [l, P, l, P, P]
+            getattr(L, "or_convolve_real_" + sfx).argtypes = [l, P, l, P, P]
             getattr(L, "or_twiddles_" + sfx).argtypes = [l, P]
             getattr(L, "or_chirp_" + sfx).argtypes = [l, i, P]
             getattr(L, "or_route_" + sfx).argtypes = [l]
@@ -134,6 +135,17 @@ def convolve(a, b):
     b = np.ascontiguousarray(b, dtype=a.dtype)
     out = np.empty(a.size + b.size - 1, dtype=a.dtype)
     rc = getattr(lib(), "or_convolve_" + _sfx(a.dtype))(a.size, _p(a), b.size, _p(b), _p(out))
+    assert rc == 0
+    return out
+
+
+def convolve_real(a, b):
+    """ConvolveReal (convolve_real.go:9-80): float32 or float64 inputs."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    out = np.empty(a.size + b.size - 1, dtype=a.dtype)
+    sfx = "c64" if a.dtype == np.float32 else "c128"
+    rc = getattr(lib(), "or_convolve_real_" + sfx)(a.size, _p(a), b.size, _p(b), _p(out))
     assert rc == 0
     return out
 

@@ -98,14 +98,18 @@ def test_convolve_real_gaussian_kernel(ctx):
 @pytest.mark.parametrize("na,nb", [(9, 6), (1000, 33), (4096, 4096), (1009, 17), (1, 1), (2, 1)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_convolve_real_matches_direct(ctx, na, nb, dtype):
-    """Random real inputs against numpy's direct float64 convolution
-    (convolve_real_test.go:58-86 checks the naive sum to 1e-3)."""
+    """Random real inputs against numpy's direct float64 convolution and the
+    oracle's ConvolveReal (convolve_real_test.go:58-86 checks the naive sum
+    to 1e-3)."""
     a = synth.uniform_real(na, (na,), dtype)
     b = synth.uniform_real(nb + 1, (nb,), dtype)
     got = A.convolve(ctx, a, b)
     ref = np.convolve(a.astype(np.float64), b.astype(np.float64))
     tol = 2e-6 * np.log2(na + nb + 2) if dtype == np.float32 else 1e-13
     assert rel_l2(got, ref) <= tol
+    # the oracle's ConvolveReal restatement in FP64 (power-of-two padding,
+    # where the device pads to an even 7-smooth length: same convolution)
+    assert rel_l2(got, O.convolve_real(a.astype(np.float64), b.astype(np.float64))) <= tol
 
 
 def test_convolve_real_errors(ctx):

@@ -48,6 +48,29 @@ def test_convolution_known_answer(dtype, tol):
     assert np.max(np.abs(got - np.array([4, 13, 22, 15]))) < tol
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-4), (np.float64, 1e-12)])
+def test_real_convolution_known_answers(dtype, tol):
+    """ConvolveReal (convolve_real.go:9-80) against the reference's own known
+    answer (convolve_real_test.go:10-29), its Gaussian smoothing case
+    (:31-56) and direct summation (:58-86, here numpy's np.convolve in FP64)
+    at lengths whose padding is and is not already a power of two."""
+    got = O.convolve_real(np.array([1, 2, 3], dtype), np.array([4, 5], dtype))
+    assert np.max(np.abs(got - np.array([4, 13, 22, 15]))) < tol
+    signal = np.zeros(32, dtype)
+    signal[8:24] = 1
+    x = np.arange(-2, 3, dtype=np.float64)
+    k = np.exp(-x * x / 2.0)
+    kernel = (k / k.sum()).astype(dtype)
+    ref = np.convolve(signal.astype(np.float64), kernel.astype(np.float64))
+    assert np.max(np.abs(O.convolve_real(signal, kernel) - ref)) < (1e-5 if dtype == np.float32 else 1e-13)
+    for na, nb in ((9, 6), (1, 1), (512, 513), (1000, 33)):
+        a = synth.uniform_real(na, (na,), dtype)
+        b = synth.uniform_real(nb + 1, (nb,), dtype)
+        ref = np.convolve(a.astype(np.float64), b.astype(np.float64))
+        err = np.linalg.norm(O.convolve_real(a, b) - ref) / np.linalg.norm(ref)
+        assert err < (2e-6 * np.log2(na + nb + 2) if dtype == np.float32 else 1e-13), (na, nb, err)
+
+
 ROUTE_SIZES = {1: [16, 256, 1024], 2: [4096, 1 << 18], 3: [6, 60, 1000, 3125], 4: [7, 49, 1009, 4093]}
 
 
