@@ -16,6 +16,7 @@ namespace afft {
 
 constexpr int GEN_THREADS = 256;      // threads per block up to 8 * 256 points per transform
 constexpr int GEN_MAXTHREADS = 512;   // threads per block above
+constexpr int GEN_ROWTHREADS = 1024;  // complex64 rows above 4096 points: one transform, up to 8 points a thread
 constexpr int GEN_MAXPASS = 16;
 
 struct GArgs {
@@ -87,7 +88,7 @@ __device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS,
 // VAR_ROW / VAR_COL / VAR_4A / VAR_4B as in kernels.hpp.  ROW-like variants
 // lay a transform out contiguously in LDS, COL variants interleave G columns.
 template <typename T, int VAR, int DIR>
-__global__ void __launch_bounds__(GEN_MAXTHREADS) k_gen(GArgs g) {
+__global__ void __launch_bounds__((VAR == VAR_ROW && sizeof(T) == 4 ? GEN_ROWTHREADS : GEN_MAXTHREADS)) k_gen(GArgs g) {
     extern __shared__ unsigned char smraw[];
     cx<T>* sm = (cx<T>*)smraw;
     const KArgs a = g.k;

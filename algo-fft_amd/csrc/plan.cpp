@@ -449,15 +449,18 @@ struct GenCfg {
 };
 
 // Two LDS buffers of G transforms, TPT threads each.  Row layouts: about 8
-// points per thread up to 256 threads, and a 512-thread block for one
-// transform above 4096 points (fewer transforms per block, so more blocks per
-// CU).  Column layouts keep about 16 points per thread: more adjacent
+// points per thread up to 256 threads, and one transform per block above
+// 4096 points: 1024 threads for complex64 (the LDS of two buffers allows one
+// block per CU, so its waves are all the latency hiding there is), 512 for
+// complex128.  Column layouts keep about 16 points per thread: more adjacent
 // columns per block make longer coalesced segments.  Measured on 512 MiB
-// batches: N=1000 rows 0.37 -> 0.26 ms, N=6000 rows 0.65 -> 0.50 ms; column
-// passes lose with fewer columns per block.
+// batches: N=1000 rows 0.37 -> 0.26 ms, N=6000 rows 0.65 -> 0.50 ms (512
+// threads) -> 0.45 ms (1024 threads); column passes lose with fewer columns
+// per block.
 static GenCfg gen_cfg(int64_t n, int prec, bool rows) {
     GenCfg c;
-    if (rows) c.tpt = n > 4096 ? GEN_MAXTHREADS : (int)std::min<int64_t>(GEN_THREADS, next_pow2((n + 7) / 8));
+    if (rows && n > 4096) c.tpt = prec == 0 ? GEN_ROWTHREADS : GEN_MAXTHREADS;
+    else if (rows) c.tpt = (int)std::min<int64_t>(GEN_THREADS, next_pow2((n + 7) / 8));
     else c.tpt = (int)std::min<int64_t>(GEN_THREADS, next_pow2((n + 15) / 16));
     c.threads = std::max(GEN_THREADS, c.tpt);
     c.g = c.threads / c.tpt;
