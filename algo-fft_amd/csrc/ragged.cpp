@@ -401,13 +401,27 @@ static int ragged_build(afft_ragged_s& r) {
     if (!tasks.empty())
         AFFT_HIP(hipMemcpy(r.tasks_dev, tasks.data(), tasks.size() * sizeof(RTask), hipMemcpyHostToDevice));
     AFFT_HIP(hipMalloc(&r.counter, RAG_COUNTERS * sizeof(int)));
+    // Each persistent launch gets a share of the device's resident blocks,
+    // so that the concurrent branches run side by side instead of the first
+    // one filling every CU until its queue drains.  Weights 4:4:6:3 (split
+    // smooth, split power-of-two, Bluestein chain, whole jobs) over the
+    // branches present, 1.7 device-fulls in all, at most one device-full per
+    // branch.  On the 4096-job mixed set this takes a graph execution from
+    // 0.600 to 0.572 ms; a set with one branch keeps the whole device.
+    static constexpr int weight[RAG_BRANCHES] = {4, 4, 6, 3};
+    bool present[RAG_BRANCHES] = {};
+    for (const RagLaunch& l : r.launches) present[rag_branch(l.kind)] = true;
+    int wsum = 0;
+    for (int b = 0; b < RAG_BRANCHES; ++b) wsum += present[b] ? weight[b] : 0;
     for (int inv = 0; inv < 2; ++inv)
         for (RagLaunch& l : r.launches) {
             const void* fn = rag_kernel(r.prec, l, inv ? 1 : -1);
             if (!fn) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no ragged kernel for a task type");
             int per_cu = 0;
             AFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, RAG_THREADS, 0));
-            l.grid[inv] = std::max(1, per_cu) * std::max(1, r.ctx->props.multiProcessorCount);
+            const int full = std::max(1, per_cu) * std::max(1, r.ctx->props.multiProcessorCount);
+            const double share = std::min(1.0, 1.7 * weight[rag_branch(l.kind)] / std::max(1, wsum));
+            l.grid[inv] = std::max(1, (int)(full * share));
         }
     AFFT_HIP(hipMalloc(&r.scratch, (size_t)std::max<int64_t>(1, r.scratch_pts) * es));
     return AFFT_OK;
