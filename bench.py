@@ -348,6 +348,8 @@ def main():
     ap.add_argument("--no-nd", action="store_true", help="skip the N-D / real / FP64 leg")
     ap.add_argument("--no-smooth", action="store_true", help="skip the smooth-length leg")
     ap.add_argument("--no-latency", action="store_true", help="skip the small-transform latency probe")
+    ap.add_argument("--prewarm", type=float, default=0.5,
+                    help="seconds of untimed steps before the warmup steps, to leave the idle clock state")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py runs on one GPU; --gpus must be 1 "
@@ -375,6 +377,15 @@ def main():
     def step():
         plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)
 
+    # Untimed: an idle GPU needs some 0.1 s of continuous work to reach its
+    # steady clocks (the first 50 steps after an idle second run about 8 %
+    # slower than the steady state), so steps run for `prewarm` seconds
+    # before the warmup steps proper.
+    t_end = time.perf_counter() + args.prewarm
+    while time.perf_counter() < t_end:
+        for _ in range(20):
+            step()
+        torch.cuda.synchronize()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -441,6 +452,7 @@ def main():
         "rel_l2_vs_fp64": rel,
         "rel_l2_budget": 2e-6 * np.log2(n),
         "wall_s_timed": round(wall, 4),
+        "prewarm_s": args.prewarm,
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(n, batch),
                      "kernel_ms": round(kms, 5), "kernel_launches_per_step": nsteps,
