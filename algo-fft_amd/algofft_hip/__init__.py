@@ -243,10 +243,22 @@ class Plan:
     def inverse(self, dst, src):
         return self._host(lib().afft_plan_inverse, dst, src)
 
+    def forward_full(self, dst, src):
+        """PlanReal2D/3D.ForwardFull: n reals in, the full n-bin spectrum out."""
+        return self._host(lib().afft_plan_forward_full, dst, src)
+
+    def inverse_full(self, dst, src):
+        """PlanReal2D/3D.InverseFull: the full n-bin spectrum in, n reals out."""
+        return self._host(lib().afft_plan_inverse_full, dst, src)
+
     # -- device-resident execution -------------------------------------------
     def exec_device(self, direction, dst_ptr, src_ptr, stream=None):
         check(lib().afft_plan_exec_device(self.h, direction, ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr),
                                           ctypes.c_void_p(stream) if stream else None))
+
+    def exec_full_device(self, direction, dst_ptr, src_ptr, stream=None):
+        check(lib().afft_plan_exec_full_device(self.h, direction, ctypes.c_void_p(dst_ptr),
+                                               ctypes.c_void_p(src_ptr), ctypes.c_void_p(stream) if stream else None))
 
     def exec_buffers(self, direction, dst, src, stream=None):
         check(lib().afft_plan_exec_buffers(self.h, direction, dst.h, src.h, stream.h if stream else None))

@@ -102,6 +102,8 @@ struct Plan {
     void* tmp[NTMP] = {nullptr, nullptr, nullptr, nullptr};
     void* stage_in = nullptr;
     void* stage_out = nullptr;
+    void* full = nullptr;             // half spectrum of the full-spectrum transforms (full.hip)
+    size_t full_bytes = 0;
     size_t stage_in_bytes = 0, stage_out_bytes = 0;
     struct HostPipe* pipe = nullptr;   // chunked host-slice execution (runtime.cpp)
     // Workspace ordering across streams: the last stream that used tmp[] and

@@ -27,6 +27,7 @@ Plan::~Plan() {
         if (t) (void)hipFree(t);
     if (stage_in) (void)hipFree(stage_in);
     if (stage_out) (void)hipFree(stage_out);
+    if (full) (void)hipFree(full);
     if (pipe) destroy_host_pipe(pipe);
     if (ws_done) (void)hipEventDestroy(ws_done);
     if (stream) (void)hipStreamDestroy(stream);

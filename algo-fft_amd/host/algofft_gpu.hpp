@@ -504,6 +504,23 @@ class DescPlan {
         if (dst.nil() || src.nil()) return ErrNilSlice;
         return impl_->Inverse(dst.data, dst.len, src.data, src.len);
     }
+    // PlanReal2D/3D.ForwardFull and InverseFull (plan_real_2d.go:231-276,
+    // :370-389; plan_real_3d.go:241-285, :392-419): real plans only; the
+    // full spectrum has as many bins as the input has points.
+    template <typename D, typename S>
+    Error ForwardFull(Slice<D> dst, Slice<const S> src) {
+        if (dst.nil() || src.nil()) return ErrNilSlice;
+        auto* h = dynamic_cast<HIPPlanImpl*>(impl_.get());
+        if (!h) return ErrNotImplemented;
+        return from_status(afft_plan_forward_full(h->handle(), dst.data, dst.len, src.data, src.len));
+    }
+    template <typename D, typename S>
+    Error InverseFull(Slice<D> dst, Slice<const S> src) {
+        if (dst.nil() || src.nil()) return ErrNilSlice;
+        auto* h = dynamic_cast<HIPPlanImpl*>(impl_.get());
+        if (!h) return ErrNotImplemented;
+        return from_status(afft_plan_inverse_full(h->handle(), dst.data, dst.len, src.data, src.len));
+    }
     Error Close() {
         impl_.reset();
         streams_.clear();

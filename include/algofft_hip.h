@@ -183,6 +183,17 @@ int afft_plan_inverse(afft_plan p, void* dst, int64_t dst_len, const void* src, 
  * wait for that execution first (outside stream capture), so executions of
  * one plan never overlap on its workspace; clones overlap freely. */
 int afft_plan_exec_device(afft_plan p, int direction, void* dst_dev, const void* src_dev, void* stream);
+/* PlanReal2D/3D.ForwardFull and InverseFull (plan_real_2d.go:231-276,
+ * :370-389; plan_real_3d.go:241-285, :392-419) for a real plan of one
+ * transform in the contiguous layout.  Forward: n reals in, the full
+ * D x ... x W spectrum of n complex bins out (bins past W/2 filled as
+ * conj(X[-k mod dims])).  Inverse: n complex bins in, of which the first
+ * W/2+1 of every row are used, n reals out.  Host slices (lengths in
+ * elements, exactly n each, else ErrLengthMismatch) or device pointers
+ * enqueued on `stream` like afft_plan_exec_device. */
+int afft_plan_forward_full(afft_plan p, void* dst, int64_t dst_len, const void* src, int64_t src_len);
+int afft_plan_inverse_full(afft_plan p, void* dst, int64_t dst_len, const void* src, int64_t src_len);
+int afft_plan_exec_full_device(afft_plan p, int direction, void* dst_dev, const void* src_dev, void* stream);
 int afft_plan_exec_buffers(afft_plan p, int direction, afft_buffer dst, afft_buffer src, afft_stream s);
 /* Waits for the work enqueued on the plan's own stream (exec calls made with
  * a NULL stream). */
@@ -245,7 +256,8 @@ int afft_ragged_destroy(afft_ragged r);
 int afft_plan_candidates(const afft_plan_desc* desc, int* strategies, int64_t* splits, int cap);
 /* The 32 cost-model features of (shape, candidate). */
 int afft_plan_features(const afft_plan_desc* desc, int strategy, int64_t split, float* out32);
-/* Average milliseconds of one forward execution (device buffers). */
+/* Milliseconds of one forward execution (device buffers): after `warmup`
+ * untimed executions, the mean of `iters` timed ones, best of three rounds. */
 int afft_tune_time(afft_context ctx, const afft_plan_desc* desc, int warmup, int iters, double* ms);
 /* Times every candidate (times[i] = ms of candidate i of
  * afft_plan_candidates, -1 if it could not run) and stores the winner as

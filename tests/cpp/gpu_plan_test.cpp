@@ -175,6 +175,44 @@ static void gpu_tests() {
         }
         CHECK(rel_l2(X, ref) < 1e-13, "2-D err %g", rel_l2(X, ref));
     }
+    // PlanReal3D.ForwardFull / InverseFull as plan_real_3d_test.go:133-228
+    // checks them (compact bins match Forward; InverseFull inverts), plus the
+    // redundant bins against direct DFTs over all three axes
+    {
+        const int64_t d = 3, h = 4, w = 6, n = d * h * w, hw = w / 2 + 1;
+        std::unique_ptr<DescPlan> rp;
+        CHECK(!NewPlanReal3D(d, h, w, &rp), "NewPlanReal3D");
+        std::vector<float> x(n), y(n);
+        for (int64_t i = 0; i < n; ++i) x[i] = std::cos(0.37f * i) - 0.1f * (i % 5);
+        std::vector<c64> X(d * h * hw), F(n);
+        CHECK(!rp->Forward(Slice<c64>(X), Slice<const float>(x)), "3-D R2C");
+        CHECK(!rp->ForwardFull(Slice<c64>(F), Slice<const float>(x)), "ForwardFull");
+        double dev = 0;
+        for (int64_t r = 0; r < d * h; ++r)
+            for (int64_t c = 0; c < hw; ++c) dev = std::max(dev, (double)std::abs(F[r * w + c] - X[r * hw + c]));
+        CHECK(dev < 1e-5, "ForwardFull compact bins differ by %g", dev);
+        std::vector<c128> ref(n);
+        for (int64_t k0 = 0; k0 < d; ++k0)
+            for (int64_t k1 = 0; k1 < h; ++k1)
+                for (int64_t k2 = 0; k2 < w; ++k2) {
+                    c128 acc = 0;
+                    for (int64_t i0 = 0; i0 < d; ++i0)
+                        for (int64_t i1 = 0; i1 < h; ++i1)
+                            for (int64_t i2 = 0; i2 < w; ++i2) {
+                                const double ph = -2 * M_PI * ((double)(k0 * i0 % d) / d + (double)(k1 * i1 % h) / h +
+                                                               (double)(k2 * i2 % w) / w);
+                                acc += (double)x[(i0 * h + i1) * w + i2] * c128(std::cos(ph), std::sin(ph));
+                            }
+                    ref[(k0 * h + k1) * w + k2] = acc;
+                }
+        CHECK(rel_l2(F, ref) < 2e-6 * 7, "ForwardFull err %g", rel_l2(F, ref));
+        CHECK(!rp->InverseFull(Slice<float>(y), Slice<const c64>(F)), "InverseFull");
+        std::vector<c64> yc(y.begin(), y.end()), xc(x.begin(), x.end());
+        CHECK(rel_l2(yc, xc) < 4e-5, "InverseFull round trip %g", rel_l2(yc, xc));
+        std::vector<c64> short_out(n - 1);
+        CHECK(rp->ForwardFull(Slice<c64>(short_out), Slice<const float>(x)).Is(ErrLengthMismatch),
+              "ForwardFull length check");
+    }
 }
 
 // Convolve / ConvolveReal / CrossCorrelate / AutoCorrelate through the mirror

@@ -117,3 +117,73 @@ def test_forward_normalized_and_unitary(ctx):
             X = np.empty((2, n // 2 + 1), np.complex128)
             p.forward(X, x)
         assert rel_l2(X, ref * s) <= C128_TOL
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (3, 4, 6), (5, 8), (6, 10), (64, 128), (16,)])
+@pytest.mark.parametrize("prec", [A.COMPLEX64, A.COMPLEX128])
+def test_forward_full_inverse_full(ctx, dims, prec):
+    """PlanReal2D/3D.ForwardFull and InverseFull (plan_real_2d.go:231-276,
+    :370-389; plan_real_3d.go:241-285, :392-419), checked as the reference's
+    TestPlanReal3D_ForwardFull / _InverseFull (plan_real_3d_test.go:133-228)
+    check them -- the first W/2+1 bins of every row are the compact
+    spectrum, InverseFull inverts ForwardFull -- and every bin, the redundant
+    ones included, against the full complex transform of the same input."""
+    rdt = np.float32 if prec == A.COMPLEX64 else np.float64
+    cdt = np.complex64 if prec == A.COMPLEX64 else np.complex128
+    x = synth.uniform_real(23, dims, rdt)
+    hw = dims[-1] // 2 + 1
+    with _rplan(ctx, dims, prec) as p:
+        X = np.empty(dims[:-1] + (hw,), cdt)
+        p.forward(X, x)
+        F = np.empty(dims, cdt)
+        p.forward_full(F, x)
+        y = np.empty_like(x)
+        p.inverse_full(y, F)
+    n = int(np.prod(dims))
+    tol = c64_tol(n) if prec == A.COMPLEX64 else C128_TOL
+    assert np.array_equal(F[..., :hw], X)
+    assert rel_l2(F, O.fftn(x.astype(np.complex128))) <= tol
+    assert rel_l2(F, np.fft.fftn(x.astype(np.float64))) <= tol
+    assert rel_l2(y, x) <= 2 * tol
+
+
+def test_full_spectrum_device_matches_host(ctx):
+    """The device-pointer form gives the host-slice form's bits."""
+    dims = (32, 48, 64)
+    n = int(np.prod(dims))
+    x = synth.uniform_real(5, dims, np.float32)
+    with _rplan(ctx, dims, A.COMPLEX64) as p:
+        F = np.empty(dims, np.complex64)
+        p.forward_full(F, x)
+        s = A.Stream(ctx)
+        bx, bF, by = A.Buffer(ctx, n // 2), A.Buffer(ctx, n), A.Buffer(ctx, n // 2)
+        try:
+            bx.upload(x.reshape(-1).view(np.complex64))   # n reals as n/2 complex words
+            p.exec_full_device(A.FORWARD, bF.ptr, bx.ptr, s.native)
+            p.exec_full_device(A.INVERSE, by.ptr, bF.ptr, s.native)
+            s.synchronize()
+            Fd = np.empty(n, np.complex64)
+            bF.download(Fd)
+            yd = np.empty(n // 2, np.complex64)
+            by.download(yd)
+        finally:
+            for b in (bx, bF, by):
+                b.close()
+            s.close()
+    assert np.array_equal(Fd.reshape(dims), F)
+    assert rel_l2(yd.view(np.float32).reshape(dims), x) <= 2 * c64_tol(n)
+
+
+def test_full_spectrum_errors(ctx):
+    x = np.zeros(32, np.float32)
+    with _rplan(ctx, (4, 8), A.COMPLEX64) as p:
+        with pytest.raises(A.ErrLengthMismatch):
+            p.forward_full(np.empty(31, np.complex64), x)
+        with pytest.raises(A.ErrLengthMismatch):
+            p.inverse_full(np.empty(32, np.float32), np.zeros(33, np.complex64))
+    with _rplan(ctx, (4, 8), A.COMPLEX64, batch=2) as p:
+        with pytest.raises(A.ErrNotImplemented):
+            p.forward_full(np.empty(64, np.complex64), np.zeros(64, np.float32))
+    with A.Plan.from_desc(ctx, (4, 8)) as p:
+        with pytest.raises(A.ErrNotImplemented):
+            p.forward_full(np.empty(32, np.complex64), x)
