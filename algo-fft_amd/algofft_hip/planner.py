@@ -106,6 +106,12 @@ def features(shape, strategy, split):
     return np.frombuffer(out, dtype=np.float32).copy()
 
 
+# plan.cpp fourstep_split: log2 of the column length where the tuning records
+# beat the balanced split, keyed by (log2 N, precision)
+TUNED_SPLIT = {(20, COMPLEX64): 9, (23, COMPLEX64): 12, (24, COMPLEX64): 13,
+               (18, COMPLEX128): 10, (19, COMPLEX128): 8, (23, COMPLEX128): 12}
+
+
 def heuristic_choice(shape, cands):
     """The built-in routing (plan.cpp emit_axis) expressed as a candidate."""
     n, prec = shape["n"], shape["precision"]
@@ -116,7 +122,8 @@ def heuristic_choice(shape, cands):
     if pow2 and n <= maxp:
         want = (STRATEGY_STOCKHAM, 0)
     elif pow2:
-        want = (STRATEGY_SIXSTEP, 1 << (int(math.log2(n)) // 2))
+        lg = int(math.log2(n))
+        want = (STRATEGY_SIXSTEP, 1 << TUNED_SPLIT.get((lg, prec), lg // 2))
     elif smooth and n <= gcap:
         want = (STRATEGY_MIXED, 0)
     elif smooth:

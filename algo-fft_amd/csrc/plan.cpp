@@ -317,6 +317,23 @@ static size_t fourstep_chunk_bytes() {
     return v;
 }
 
+// The four-step split n1 (column length) of a power-of-two length 2^lg: the
+// balanced one, except where the gfx950 tuning records show another one
+// faster by more than 5 % on average over their shapes of that length:
+// complex64 2^20 as 512 x 2048 (7 %), 2^23 as 4096 x 2048 (44 %), 2^24 as
+// 8192 x 2048 (7 %); complex128 2^18 as 1024 x 256 (13 %), 2^19 as
+// 256 x 2048 (10 %), 2^23 as 4096 x 2048 (32 %).
+static int64_t fourstep_split(int prec, int lg) {
+    static constexpr int tuned[2][25] = {
+        // complex64, by lg
+        {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 9, 0, 0, 12, 13},
+        // complex128
+        {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10, 8, 0, 0, 0, 12, 0},
+    };
+    const int t = lg >= 0 && lg < 25 ? tuned[prec ? 1 : 0][lg] : 0;
+    return 1LL << (t ? t : lg / 2);
+}
+
 static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int prec = p.desc.precision;
     const int dir = di ? 1 : -1;
@@ -325,7 +342,7 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
         return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step needs a unit-stride power-of-two axis");
     const int lg = ilog2(j.n);
     const bool mf = p.desc.flags & AFFT_FLAG_MFMA;
-    int64_t n1 = 1LL << (lg / 2);
+    int64_t n1 = fourstep_split(prec, lg);
     if (mf) {
         // the MFMA passes have lengths 256 and 1024
         n1 = 0;
