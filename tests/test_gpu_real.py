@@ -1,6 +1,8 @@
 """Real transforms: PlanRealT Forward/Inverse (plan_real_generic.go:142-355),
-PlanReal2D/3D (plan_real_3d.go:145-226) against the oracle restatements
-(R2C by packing + recombination, C2R by the closed-form repack).
+PlanReal2D/3D (plan_real_3d.go:145-226) and their ForwardFull/InverseFull
+(plan_real_2d.go:231-389, plan_real_3d.go:241-419) against the oracle
+restatements (R2C by packing + recombination, C2R by the closed-form repack,
+the full spectrum by the complex N-D transform).
 
 Budgets against the FP64 oracle: float32 rel L2 <= 2e-6*log2(N);
 float64 rel L2 <= 1e-13.
