@@ -160,3 +160,27 @@ def test_nd_c2c(ctx, dims):
     n = int(np.prod(dims))
     assert rel_l2(out, ref) <= c64_tol(n)
     assert rel_l2(back, x) <= 2 * c64_tol(n)
+
+
+TUNED_SPLITS = {(20, A.COMPLEX64): 512, (23, A.COMPLEX64): 4096, (24, A.COMPLEX64): 8192,
+                (18, A.COMPLEX128): 1024, (19, A.COMPLEX128): 256, (23, A.COMPLEX128): 4096}
+
+
+@pytest.mark.parametrize("lg,prec", sorted(TUNED_SPLITS))
+def test_tuned_fourstep_splits(ctx, lg, prec):
+    """The lengths whose default four-step split comes from the tuning
+    records: the plan takes that split, matches numpy's FP64 FFT and its
+    inverse returns the input."""
+    n = 1 << lg
+    split = TUNED_SPLITS[(lg, prec)]
+    cdt = np.complex64 if prec == A.COMPLEX64 else np.complex128
+    x = synth.uniform_complex(lg, (n,), cdt)
+    with A.Plan.from_desc(ctx, (n,), precision=prec) as p:
+        assert p.describe() == f"hip_fourstep{split}x{n // split}"
+        X = np.empty_like(x)
+        p.forward(X, x)
+        y = np.empty_like(x)
+        p.inverse(y, X)
+    tol = c64_tol(n) if prec == A.COMPLEX64 else C128_TOL
+    assert rel_l2(X, np.fft.fft(x.astype(np.complex128))) <= tol
+    assert rel_l2(y, x) <= 2 * tol
