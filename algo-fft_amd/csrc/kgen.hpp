@@ -2,13 +2,14 @@
 // path; the reference's mixedradix.go:45-358 does the same recursion
 // depth-first on the CPU and sends 7-smooth sizes to Bluestein instead).
 //
-// One block of 256 threads (512 for transforms above 2048 points) holds
-// G = blockDim.x / TPT transforms, TPT threads each, about 8 points per
-// thread.  Every pass
-// reads its butterfly inputs from one LDS buffer (global memory for the first
-// pass), applies the per-pass twiddles, runs the radix-R codelet and writes
-// to the other LDS buffer (global memory for the last pass, whose output
-// index j + r*Ns is already natural order).  One barrier per pass.
+// A block of 256 threads holds G = blockDim.x / TPT transforms, TPT threads
+// each: about 8 points per thread for rows, 16 for columns.  Rows above 4096
+// points take one transform per block of 1024 threads (complex64) or 512
+// (complex128).  Every pass reads its butterfly inputs from one LDS buffer
+// (global memory for the first pass), applies the per-pass twiddles, runs
+// the radix-R codelet and writes to the other LDS buffer (global memory for
+// the last pass, whose output index j + r*Ns is already natural order).  One
+// barrier per pass.
 #pragma once
 #include "kernels.hpp"
 
