@@ -227,9 +227,26 @@ class Plan:
         return lib().afft_plan_workspace_bytes(self.h)
 
     # -- host-slice execution (Forward/Inverse with []complex) --------------
-    def _host(self, fn, dst, src):
+    @property
+    def kind(self):
+        return self.desc.kind if self.desc is not None else C2C
+
+    def _host(self, fn, dst, src, dst_real, src_real):
+        """The C entry points count elements only, so the element type of each
+        side is checked here: complex64/float32 for complex64 plans,
+        complex128/float64 for complex128 plans, real or complex as the
+        plan's kind has it for that direction.  A wrong type is
+        ErrNotImplemented, as for a wrong dynamic type in gpu/plan.go."""
         if dst is None or src is None:
             raise ErrNilSlice("dst or src is nil")  # noqa: F821
+        if not isinstance(dst, np.ndarray) or not isinstance(src, np.ndarray):
+            raise ErrNotImplemented("dst and src must be numpy arrays")  # noqa: F821
+        prec = self.desc.precision if self.desc is not None else self.precision
+        for arr, real, side in ((dst, dst_real, "dst"), (src, src_real, "src")):
+            want = {(COMPLEX64, False): np.complex64, (COMPLEX64, True): np.float32,
+                    (COMPLEX128, False): np.complex128, (COMPLEX128, True): np.float64}[(prec, real)]
+            if arr.dtype != want:
+                raise ErrNotImplemented(f"{side} has element type {arr.dtype}, the plan needs {np.dtype(want)}")  # noqa: F821
         if not dst.flags.c_contiguous:
             raise ValueError("dst must be C-contiguous")
         src = np.ascontiguousarray(src)
@@ -238,18 +255,20 @@ class Plan:
         return dst
 
     def forward(self, dst, src):
-        return self._host(lib().afft_plan_forward, dst, src)
+        real = self.kind in (R2C, C2R)   # real plans: reals in, half spectrum out
+        return self._host(lib().afft_plan_forward, dst, src, False, real)
 
     def inverse(self, dst, src):
-        return self._host(lib().afft_plan_inverse, dst, src)
+        real = self.kind in (R2C, C2R)
+        return self._host(lib().afft_plan_inverse, dst, src, real, False)
 
     def forward_full(self, dst, src):
         """PlanReal2D/3D.ForwardFull: n reals in, the full n-bin spectrum out."""
-        return self._host(lib().afft_plan_forward_full, dst, src)
+        return self._host(lib().afft_plan_forward_full, dst, src, False, True)
 
     def inverse_full(self, dst, src):
         """PlanReal2D/3D.InverseFull: the full n-bin spectrum in, n reals out."""
-        return self._host(lib().afft_plan_inverse_full, dst, src)
+        return self._host(lib().afft_plan_inverse_full, dst, src, True, False)
 
     # -- device-resident execution -------------------------------------------
     def exec_device(self, direction, dst_ptr, src_ptr, stream=None):
@@ -315,7 +334,8 @@ class FastPlan:
 class Ragged:
     """A ragged batch: complex transforms of lengths `sizes` packed back to
     back, run by one persistent launch (exec), its captured graph
-    (exec_graph) or one plan execution per job (exec_per_size)."""
+    (exec_graph), one batched plan per run of equal lengths (exec_per_size)
+    or one plan execution per job (exec_per_job)."""
 
     def __init__(self, ctx, sizes, precision=COMPLEX64, flags=0):
         arr = (ctypes.c_int64 * len(sizes))(*[int(v) for v in sizes])
@@ -344,6 +364,9 @@ class Ragged:
 
     def exec_per_size(self, direction, dst_ptr, src_ptr, stream=None):
         self._x(lib().afft_ragged_exec_per_size, direction, dst_ptr, src_ptr, stream)
+
+    def exec_per_job(self, direction, dst_ptr, src_ptr, stream=None):
+        self._x(lib().afft_ragged_exec_per_job, direction, dst_ptr, src_ptr, stream)
 
     def close(self):
         if self.h:

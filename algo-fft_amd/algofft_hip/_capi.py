@@ -82,6 +82,7 @@ SIGNATURES = [
     ("afft_ragged_exec", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("afft_ragged_exec_graph", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("afft_ragged_exec_per_size", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("afft_ragged_exec_per_job", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("afft_ragged_destroy", c_int, [c_void_p]),
     ("afft_plan_candidates", c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(c_int), ctypes.POINTER(c_i64), c_int]),
     ("afft_plan_features", c_int, [ctypes.POINTER(PlanDesc), c_int, c_i64, ctypes.POINTER(ctypes.c_float)]),

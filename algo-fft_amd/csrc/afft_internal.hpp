@@ -150,6 +150,9 @@ bool parse_strategy(const std::string& algo, Candidate* c);
 // Predicted log-runtimes of candidates (GPU inference); 0 on success.
 int mlp_predict_host(Mlp* m, const float* x, int64_t n, float* y);
 int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s);
+// The steps of direction di (0 forward, 1 inverse) with explicit workspace
+// buffers; allocates nothing and records no event.
+int launch_steps(Plan& p, int di, void* dst, const void* src, void* const* tmp, hipStream_t s);
 
 // FP64 roots of unity W_L^m = exp(-2πi m / L), exact integer reduction of m.
 void root_of_unity(int64_t m, int64_t L, double* re, double* im);

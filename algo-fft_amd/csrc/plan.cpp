@@ -557,6 +557,7 @@ static int emit_gen_single(Plan& p, int di, const AxisJob& j, std::string& algo)
     const int prec = p.desc.precision;
     const int dir = di ? 1 : -1;
     const int es = prec ? 16 : 8;
+    if (p.desc.flags & AFFT_FLAG_MFMA) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "mixed radix has no MFMA form");
     int var;
     int64_t count = j.count, slices = j.slices, in_bd = j.in_bd, out_bd = j.out_bd;
     if (j.in_nd == 1 && j.out_nd == 1) var = VAR_ROW;
@@ -942,8 +943,10 @@ static bool needs_rows(const Plan& p, int64_t n, int strategy) {
 static int emit_axis_direct(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int prec = p.desc.precision;
     const bool smooth = !smooth_schedule(j.n).empty();
-    if ((p.desc.flags & AFFT_FLAG_MFMA) && !is_pow2(j.n))
-        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "the MFMA form covers powers of two only");
+    if ((p.desc.flags & AFFT_FLAG_MFMA) &&
+        (!is_pow2(j.n) || j.strategy == AFFT_STRATEGY_MIXED || j.strategy == AFFT_STRATEGY_BLUESTEIN))
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED,
+                         "the MFMA form covers powers of two on the Stockham and four-step routes only");
     switch (j.strategy) {
     case AFFT_STRATEGY_AUTO: break;
     case AFFT_STRATEGY_STOCKHAM:
@@ -1154,8 +1157,12 @@ static int emit_real_rows(Plan& p, int di, int64_t n, int64_t rows, int64_t slic
 // Planner's choice for AFFT_STRATEGY_AUTO on a 1-D axis: wisdom for this
 // size first, then the context's cost model, else the built-in routing
 // (cf. planner.go:43-78: registry, wisdom, heuristic).
+// MFMA plans keep the MFMA routing: wisdom and the cost model rank the
+// plain kernels, and their picks (mixed radix, Bluestein, other four-step
+// splits) have no MFMA form.
 void choose_strategy(Plan& p, AxisJob& j) {
     if (j.in_nd != 1 || j.out_nd != 1) return;
+    if (p.desc.flags & AFFT_FLAG_MFMA) return;
     Context* c = p.ctx;
     std::lock_guard<std::mutex> lock(c->mu);
     auto it = c->wisdom.find(std::make_tuple(j.n, p.desc.precision, c->features()));
@@ -1312,6 +1319,21 @@ static int build_plan_steps(Plan& p) {
     return AFFT_OK;
 }
 
+// Enqueues the steps of direction di on s, with the roles resolved to src,
+// dst and the workspace buffers tmp[0 .. NTMP).
+int launch_steps(Plan& p, int di, void* dst, const void* src, void* const* tmp, hipStream_t s) {
+    void* base[ROLE_COUNT] = {const_cast<void*>(src), dst, tmp[0], tmp[1], tmp[2], tmp[3]};
+    for (Step& st : p.steps[di]) {
+        char* ip = (char*)base[st.in_role] + st.in_off;
+        char* op = (char*)base[st.out_role] + st.out_off;
+        std::memcpy(st.arg.data() + st.in_field, &ip, sizeof(void*));
+        std::memcpy(st.arg.data() + st.out_field, &op, sizeof(void*));
+        void* args[] = {st.arg.data()};
+        AFFT_HIP(hipLaunchKernel(st.fn, st.grid, st.block, args, (size_t)st.lds_bytes, s));
+    }
+    return AFFT_OK;
+}
+
 // A plan's workspace (tmp[]) is shared by all its executions.  When one
 // execution is enqueued on a different stream than the previous one, the new
 // stream first waits for the previous use to finish, so a plan may be driven
@@ -1336,15 +1358,8 @@ int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s) {
             if (p.ws_used && p.ws_stream != s) AFFT_HIP(hipStreamWaitEvent(s, p.ws_done, 0));
         }
     }
-    void* base[ROLE_COUNT] = {const_cast<void*>(src), dst, p.tmp[0], p.tmp[1], p.tmp[2], p.tmp[3]};
-    for (Step& st : p.steps[di]) {
-        char* ip = (char*)base[st.in_role] + st.in_off;
-        char* op = (char*)base[st.out_role] + st.out_off;
-        std::memcpy(st.arg.data() + st.in_field, &ip, sizeof(void*));
-        std::memcpy(st.arg.data() + st.out_field, &op, sizeof(void*));
-        void* args[] = {st.arg.data()};
-        AFFT_HIP(hipLaunchKernel(st.fn, st.grid, st.block, args, (size_t)st.lds_bytes, s));
-    }
+    int rc = launch_steps(p, di, dst, src, p.tmp, s);
+    if (rc) return rc;
     if (ws) {
         AFFT_HIP(hipEventRecord(p.ws_done, s));
         p.ws_stream = s;

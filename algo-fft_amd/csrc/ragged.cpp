@@ -10,9 +10,13 @@
 //                           counter-reset kernel
 //   afft_ragged_exec_graph  the same launches captured once in a hipGraph
 //                           and replayed as one graph launch
-//   afft_ragged_exec_per_size  every job through the plan of its own size,
-//                           one plan execution per job: the launch-per-size
-//                           form the persistent kernel is measured against
+//   afft_ragged_exec_per_size  one batched plan execution per run of
+//                           consecutive jobs of equal length: with the jobs
+//                           sorted by length, one launch sequence per
+//                           distinct size (the plain per-size path the
+//                           persistent kernel is measured against)
+//   afft_ragged_exec_per_job   every job through the plan of its own size,
+//                           one plan execution per job
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -90,7 +94,9 @@ struct afft_ragged_s {
     hipStream_t last = nullptr;
     bool ran = false;
     std::map<std::tuple<int, void*, const void*, void*>, hipGraphExec_t> graphs;
-    std::map<int64_t, afft_plan> per_size;
+    // plain plans of the comparison forms, by (length, batch)
+    std::map<std::pair<int64_t, int64_t>, afft_plan> per_job;
+    std::map<std::pair<int64_t, int64_t>, afft_plan> per_size;
     std::mutex mu;
     ~afft_ragged_s();
 };
@@ -98,6 +104,7 @@ struct afft_ragged_s {
 afft_ragged_s::~afft_ragged_s() {
     if (ctx) (void)hipSetDevice(ctx->device);
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : per_job) afft_plan_destroy(kv.second);
     for (auto& kv : per_size) afft_plan_destroy(kv.second);
     if (jobs_dev) (void)hipFree(jobs_dev);
     if (tasks_dev) (void)hipFree(tasks_dev);
@@ -570,31 +577,69 @@ int afft_ragged_exec_graph(afft_ragged r, int direction, void* dst, const void* 
     return AFFT_OK;
 }
 
+// A plain C2C plan of length n over `batch` contiguous transforms, cached.
+static int ragged_plain_plan(afft_ragged_s& r, std::map<std::pair<int64_t, int64_t>, afft_plan>& cache, int64_t n,
+                             int64_t batch, afft_plan* out) {
+    auto it = cache.find({n, batch});
+    if (it == cache.end()) {
+        afft_plan_desc d{};
+        d.kind = AFFT_C2C;
+        d.precision = r.prec;
+        d.rank = 1;
+        d.dims[0] = n;
+        d.batch = batch;
+        d.flags = r.flags;
+        afft_plan p = nullptr;
+        int rc = afft_plan_create_desc(r.ctx, &d, &p);
+        if (rc) return rc;
+        it = cache.emplace(std::make_pair(n, batch), p).first;
+    }
+    *out = it->second;
+    return AFFT_OK;
+}
+
 int afft_ragged_exec_per_size(afft_ragged r, int direction, void* dst, const void* src, void* stream) {
     if (!r) return set_error(AFFT_ERR_INVALID_ARGUMENT, "NULL handle");
     if (!dst || !src) return set_error(AFFT_ERR_NIL_SLICE, "dst or src is nil");
+    if (direction != AFFT_FORWARD && direction != AFFT_INVERSE)
+        return set_error(AFFT_ERR_INVALID_ARGUMENT, "direction must be -1 or +1");
+    std::lock_guard<std::mutex> l(r->mu);
+    AFFT_HIP(hipSetDevice(r->ctx->device));
+    const int es = r->prec ? 16 : 8;
+    const size_t nj = r->sizes.size();
+    for (size_t i = 0; i < nj;) {
+        // the run of consecutive jobs of this length is one contiguous batch
+        size_t e = i + 1;
+        while (e < nj && r->sizes[e] == r->sizes[i]) ++e;
+        afft_plan p = nullptr;
+        int rc = ragged_plain_plan(*r, r->per_size, r->sizes[i], (int64_t)(e - i), &p);
+        if (rc) return rc;
+        const int64_t o = r->offs[i] * es;
+        rc = afft_plan_exec_device(p, direction, (char*)dst + o, (const char*)src + o,
+                                   stream ? stream : (void*)r->stream);
+        if (rc) return rc;
+        i = e;
+    }
+    clear_error();
+    return AFFT_OK;
+}
+
+int afft_ragged_exec_per_job(afft_ragged r, int direction, void* dst, const void* src, void* stream) {
+    if (!r) return set_error(AFFT_ERR_INVALID_ARGUMENT, "NULL handle");
+    if (!dst || !src) return set_error(AFFT_ERR_NIL_SLICE, "dst or src is nil");
+    if (direction != AFFT_FORWARD && direction != AFFT_INVERSE)
+        return set_error(AFFT_ERR_INVALID_ARGUMENT, "direction must be -1 or +1");
     std::lock_guard<std::mutex> l(r->mu);
     AFFT_HIP(hipSetDevice(r->ctx->device));
     const int es = r->prec ? 16 : 8;
     for (size_t i = 0; i < r->sizes.size(); ++i) {
         const int64_t n = r->sizes[i];
-        auto it = r->per_size.find(n);
-        if (it == r->per_size.end()) {
-            afft_plan_desc d{};
-            d.kind = AFFT_C2C;
-            d.precision = r->prec;
-            d.rank = 1;
-            d.dims[0] = n;
-            d.batch = 1;
-            d.flags = r->flags;
-            afft_plan p = nullptr;
-            int rc = afft_plan_create_desc(r->ctx, &d, &p);
-            if (rc) return rc;
-            it = r->per_size.emplace(n, p).first;
-        }
+        afft_plan p = nullptr;
+        int rc = ragged_plain_plan(*r, r->per_job, n, 1, &p);
+        if (rc) return rc;
         const int64_t o = r->offs[i] * es;
-        int rc = afft_plan_exec_device(it->second, direction, (char*)dst + o, (const char*)src + o,
-                                       stream ? stream : (void*)r->stream);
+        rc = afft_plan_exec_device(p, direction, (char*)dst + o, (const char*)src + o,
+                                   stream ? stream : (void*)r->stream);
         if (rc) return rc;
     }
     clear_error();

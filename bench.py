@@ -270,13 +270,15 @@ def host_leg(torch, A, ctx, n, batch, reps=3):
 def ragged_leg(torch, A, ctx, stream, steps):
     """Mixed-size workload: 4096 complex64 transforms, lengths log-uniform in
     [16, 65536] (60% powers of two, 30% 7-smooth, 10% primes via Bluestein),
-    packed back to back.  Persistent work-queue launch and its graph replay
-    against one plan execution per job."""
+    packed back to back.  The persistent work-queue launch and its graph
+    replay against the plain per-size path: one batched plan per distinct
+    length (Plan.ForwardBatch per size) on a copy sorted by length, and one
+    plan execution per job.  Accuracy: every job of the graph output against
+    numpy's FP64 FFT, as rel L2 over the complex64 budget 2e-6*log2(n)."""
     from algofft_hip import synth
     sizes = synth.ragged_sizes(1234, 4096)
-    rg = A.Ragged(ctx, sizes, A.COMPLEX64)
-    total = len(rg)
     g = torch.Generator(device="cuda:0").manual_seed(5)
+    total = sum(sizes)
     x = torch.complex(torch.rand(total, device="cuda:0", generator=g) * 2 - 1,
                       torch.rand(total, device="cuda:0", generator=g) * 2 - 1)
     y = torch.empty_like(x)
@@ -293,19 +295,68 @@ def ragged_leg(torch, A, ctx, stream, steps):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / k
 
+    t0 = time.perf_counter()
+    rg = A.Ragged(ctx, sizes, A.COMPLEX64)
+    create_s = time.perf_counter() - t0
     ms_p = timed(lambda: rg.exec(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), steps)
     ms_g = timed(lambda: rg.exec_graph(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), steps)
-    ms_s = timed(lambda: rg.exec_per_size(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), max(2, steps // 10))
+    ms_j = timed(lambda: rg.exec_per_job(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), max(2, steps // 10))
+    rg.exec_graph(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)
+    torch.cuda.synchronize()
+    # accuracy of every job of the graph output
+    xs, ys = x.cpu().numpy(), y.cpu().numpy()
+    ratios, off = [], 0
+    for n in sizes:
+        ref = np.fft.fft(xs[off:off + n].astype(np.complex128))
+        got = ys[off:off + n].astype(np.complex128)
+        ratios.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)) / (2e-6 * max(1.0, np.log2(n))))
+        off += n
+    rg.close()
+    # the same jobs sorted by length: the per-size comparator's layout (one
+    # contiguous batch per distinct length), and the graph on that layout
+    order = sorted(range(len(sizes)), key=lambda i: (sizes[i], i))
+    ssizes = [sizes[i] for i in order]
+    rs = A.Ragged(ctx, ssizes, A.COMPLEX64)
+    ms_gs = timed(lambda: rs.exec_graph(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), steps)
+    ms_s = timed(lambda: rs.exec_per_size(A.FORWARD, y.data_ptr(), x.data_ptr(), sh), max(3, steps // 2))
+    rs.close()
     flops = float(sum(5.0 * n * np.log2(n) for n in sizes))
     alg = 16.0 * total
-    rg.close()
+    del x, y
+    torch.cuda.empty_cache()
     return {"workload": "4096 ragged complex64 forward transforms, N log-uniform 16..65536 "
                         "(60% pow2 / 30% 7-smooth / 10% prime)",
-            "points": total, "distinct_sizes": len(set(sizes)),
-            "persistent_ms": round(ms_p, 4), "graph_ms": round(ms_g, 4), "per_size_launch_ms": round(ms_s, 4),
+            "points": total, "distinct_sizes": len(set(sizes)), "create_s": round(create_s, 3),
+            "persistent_ms": round(ms_p, 4), "graph_ms": round(ms_g, 4),
+            "graph_sorted_ms": round(ms_gs, 4),
+            "per_size_batched_ms": round(ms_s, 4), "per_size_batched_launch_groups": len(set(sizes)),
+            "per_job_ms": round(ms_j, 4),
             "gflops_graph": round(flops / (ms_g * 1e-3) / 1e9, 1),
             "gbs_graph_algorithmic": round(alg / (ms_g * 1e-3) / 1e9, 1),
-            "speedup_graph_vs_per_size": round(ms_s / ms_g, 2)}
+            "speedup_graph_vs_per_size_batched": round(ms_s / ms_g, 2),
+            "speedup_graph_vs_per_job": round(ms_j / ms_g, 2),
+            "rel_l2_over_budget_max": round(max(ratios), 5),
+            "rel_l2_over_budget_mean": round(float(np.mean(ratios)), 5),
+            "jobs_checked": len(ratios)}
+
+
+def plan_create_leg(A, ctx, reps=5):
+    """Host time of plan creation (NewPlan; the reference publishes 4.48 ms
+    for NewPlan at N=65536, BENCHMARKS.md:129-131): complex64 N=65536,
+    complex128 Bluestein N=1,000,003 (filter spectra of the padded 2^21
+    points), median of `reps` creations each."""
+    out = {}
+    for name, dims, prec in (("complex64 N=65536", (65536,), A.COMPLEX64),
+                             ("complex64 N=4096 batch 16384", (4096,), A.COMPLEX64),
+                             ("complex128 Bluestein N=1000003", (1_000_003,), A.COMPLEX128)):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            p = A.Plan.from_desc(ctx, dims, precision=prec, batch=16384 if dims == (4096,) else 1)
+            ts.append(time.perf_counter() - t0)
+            p.close()
+        out[name] = {"ms_median": round(1e3 * float(np.median(ts)), 3), "ms_min": round(1e3 * min(ts), 3)}
+    return out
 
 
 def cpu_baseline(n, seconds=10.0):
@@ -315,7 +366,10 @@ def cpu_baseline(n, seconds=10.0):
     import pyoracle as O
     from algofft_hip import synth
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # the GPU box allots 16 host CPUs to one GPU, whatever the machine has:
+    # use them all, or every CPU this process may run on if that is fewer
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, avail))
     sample = 2048
     x = synth.uniform_complex(99, (sample, n), np.complex64)
     O.fft_batch(x[:threads], nthreads=threads)  # warm
@@ -327,7 +381,17 @@ def cpu_baseline(n, seconds=10.0):
         if el >= seconds:
             break
     flops = 5.0 * n * np.log2(n) * sample * reps
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": round(flops / el / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpus_available": avail, "threads": threads, "cpu_model": model,
             "sample": f"{reps}x ForwardBatch of {sample} complex64 N={n} transforms "
                       f"({el:.1f} s, {threads} threads, oracle routes N={n} to "
                       f"{['copy', 'radix-2 Stockham', 'six-step', 'mixed radix', 'Bluestein'][O.route(n)]})"}
@@ -348,6 +412,7 @@ def main():
     ap.add_argument("--no-nd", action="store_true", help="skip the N-D / real / FP64 leg")
     ap.add_argument("--no-smooth", action="store_true", help="skip the smooth-length leg")
     ap.add_argument("--no-latency", action="store_true", help="skip the small-transform latency probe")
+    ap.add_argument("--no-create", action="store_true", help="skip the plan-creation timing leg")
     ap.add_argument("--prewarm", type=float, default=0.5,
                     help="seconds of untimed steps before the warmup steps, to leave the idle clock state")
     args = ap.parse_args()
@@ -455,6 +520,8 @@ def main():
         "prewarm_s": args.prewarm,
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(n, batch),
+                     "frac_source": "mean of per-launch HIP event pairs on the launch stream",
+                     "frac_from_ms_per_step": round(alg_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "kernel_ms": round(kms, 5), "kernel_launches_per_step": nsteps,
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "device_copy_gbs": round(copy_gbs, 1), "frac_of_device_copy": round(gbs / copy_gbs, 4)},
@@ -477,6 +544,8 @@ def main():
         out["latency_small_n"] = latency_leg()
     if not args.no_host:
         out["host_end_to_end"] = host_leg(torch, A, ctx, n, batch)
+    if not args.no_create:
+        out["plan_create"] = plan_create_leg(A, ctx)
     ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)

@@ -242,8 +242,13 @@ int afft_ragged_exec(afft_ragged r, int direction, void* dst, const void* src, v
 /* The same launch captured once per (direction, dst, src, stream) in a
  * hipGraph and replayed. */
 int afft_ragged_exec_graph(afft_ragged r, int direction, void* dst, const void* src, void* stream);
-/* Every job through the plan of its own size: one plan execution per job. */
+/* The plain per-size path: one batched plan execution (Plan.ForwardBatch,
+ * plan.go:504-530) per run of consecutive jobs of equal length.  With the
+ * jobs sorted by length that is one batched plan per distinct size. */
 int afft_ragged_exec_per_size(afft_ragged r, int direction, void* dst, const void* src, void* stream);
+/* Every job through a batch-1 plan of its own size (Plan.Forward,
+ * plan.go:332-369): one plan execution per job. */
+int afft_ragged_exec_per_job(afft_ragged r, int direction, void* dst, const void* src, void* stream);
 int afft_ragged_destroy(afft_ragged r);
 
 /* ---- planning: candidates, autotuner, wisdom -------------------------------

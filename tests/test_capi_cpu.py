@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import algofft_hip as A
@@ -75,3 +76,23 @@ def test_fast_plan_length_checks_come_first():
         assert A.lib().afft_fast_plan_create(None, n, 1, A.COMPLEX64, None, ctypes.byref(h)) == 4
     assert A.lib().afft_fast_plan_create(None, 1024, 0, A.COMPLEX64, None, ctypes.byref(h)) == 4
     assert A.lib().afft_fast_plan_create(None, 1024, 1, A.COMPLEX64, None, ctypes.byref(h)) == 10
+
+
+@pytest.mark.parametrize("kind,prec,call,dst,src", [
+    (A.C2C, A.COMPLEX128, "forward", np.complex128, np.complex64),
+    (A.C2C, A.COMPLEX64, "inverse", np.complex128, np.complex64),
+    (A.R2C, A.COMPLEX64, "forward", np.complex64, np.float64),
+    (A.R2C, A.COMPLEX64, "inverse", np.complex64, np.complex64),
+    (A.R2C, A.COMPLEX128, "forward_full", np.complex128, np.complex128),
+    (A.R2C, A.COMPLEX64, "inverse_full", np.float64, np.complex64),
+])
+def test_host_slices_of_the_wrong_element_type_are_refused(kind, prec, call, dst, src):
+    """The host-slice entry points count elements, not bytes: an array of the
+    wrong element type is refused with ErrNotImplemented before any copy
+    (a wrong dynamic type in gpu/plan.go's type switch)."""
+    d = _capi.PlanDesc()
+    d.kind, d.precision, d.rank = kind, prec, 1
+    d.dims[0] = 64
+    p = A.Plan(None, d)   # the check runs before the library is called
+    with pytest.raises(A.ErrNotImplemented):
+        getattr(p, call)(np.zeros(64, dst), np.zeros(64, src))

@@ -76,6 +76,22 @@ def test_batch_1024x256_equals_loop(ctx):
     assert rel_l2(got, ref) <= c64_tol(n)
 
 
+def test_c1_inverse_1024x256_and_round_trip(ctx):
+    # the plain CPU-runnable workload in the other direction: InverseBatch of
+    # 256 transforms of 1024 points against the oracle's inverse (1/N
+    # scaling, plan.go:545-571), and the forward-inverse round trip
+    n, batch = 1024, 256
+    x = synth.uniform_complex(1, (batch, n), np.complex64)
+    got, _ = _run(ctx, x, inverse=True)
+    ref = O.fft_batch(x.astype(np.complex128), inverse=True)
+    err = rel_l2(got, ref)
+    print(f"inverse n={n} batch={batch} relL2={err:.3e}")
+    assert err <= c64_tol(n)
+    X, _ = _run(ctx, x)
+    back, _ = _run(ctx, X, inverse=True)
+    assert rel_l2(back, x) <= 2 * c64_tol(n)
+
+
 def test_in_place(ctx):
     n = 4096
     x = synth.uniform_complex(3, (4, n), np.complex64)

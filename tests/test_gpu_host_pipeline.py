@@ -2,7 +2,9 @@
 slices, gpu/plan.go:100-143): chunks of whole transforms pipelined over two
 streams.  Results must equal the device-resident execution of the same plan
 bit for bit (each transform runs the same kernel), from pageable and from
-page-locked host memory, with a tail chunk shorter than the others."""
+page-locked host memory, with a tail chunk shorter than the others; rows on
+both sides of every chunk boundary and the rows of the tail chunk are also
+checked against the oracle."""
 import ctypes
 
 import numpy as np
@@ -11,8 +13,19 @@ import torch
 
 import algofft_hip as A
 from algofft_hip import synth
+import pyoracle as O
+from helpers import c64_tol, rel_l2
+
+# rows of the 4100-row batch around the boundaries of its 512-row chunks,
+# and the four rows of the tail chunk
+CHUNK_ROWS = sorted({0, 1, 4095, 4096, 4097, 4099} | {k * 512 + d for k in range(1, 8) for d in (-1, 0)})
+
 
 pytestmark = pytest.mark.gpu
+
+
+def _oracle_rows(x, rows, inverse=False):
+    return O.fft_batch(x[rows].astype(np.complex128), inverse=inverse)
 
 
 def _device_result(plan, direction, x):
@@ -63,6 +76,8 @@ def test_c2c_batch_pipeline_matches_device(ctx, pinned):
         want_i = _device_result(p, A.INVERSE, x).reshape(batch, n)
     assert np.array_equal(got, want_f)
     assert np.array_equal(got_inv, want_i)
+    assert rel_l2(got[CHUNK_ROWS], _oracle_rows(x, CHUNK_ROWS)) <= c64_tol(n)
+    assert rel_l2(got_inv[CHUNK_ROWS], _oracle_rows(x, CHUNK_ROWS, inverse=True)) <= c64_tol(n)
 
 
 def test_r2c_batch_pipeline_matches_device(ctx):
@@ -77,6 +92,13 @@ def test_r2c_batch_pipeline_matches_device(ctx):
         got = np.empty((batch, n // 2 + 1), np.complex64)
         p.forward(got, x)
     assert np.array_equal(got, yt.cpu().numpy())
+    # chunk boundaries and the tail chunk against the oracle's real-FFT
+    # restatement (plan_real_generic.go:142-228); chunks hold 16 MiB of the
+    # larger side: 1023 rows of 2049 bins, then a tail of 16 rows
+    chunk = (16 << 20) // max(4 * n, 8 * (n // 2 + 1))
+    rows = sorted({0, batch - 1} | {k * chunk + d for k in range(1, batch // chunk + 1) for d in (-1, 0)})
+    ref = np.stack([O.rfft(x[r].astype(np.float64)) for r in rows])
+    assert rel_l2(got[rows], ref) <= c64_tol(n)
 
 
 def test_small_batches_take_the_direct_path(ctx):

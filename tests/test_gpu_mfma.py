@@ -65,3 +65,30 @@ def test_mfma_unsupported_shapes_are_reported(ctx):
             A.Plan.from_desc(ctx, (n,), flags=A.FLAG_MFMA)
     with pytest.raises(A.ErrNotImplemented):
         A.Plan.from_desc(ctx, (4096,), precision=A.COMPLEX128, flags=A.FLAG_MFMA)
+
+
+def test_mfma_plans_ignore_wisdom_for_the_plain_kernels():
+    """Wisdom ranks the plain kernels: an imported pick of a four-step split
+    the MFMA form does not build (2^20 as 512 x 2048), or of the mixed-radix
+    kernel for 4096, must not reach an MFMA plan.  The plan keeps the MFMA
+    route and its result stays within budget; a plain plan follows the
+    wisdom."""
+    with A.Context(0) as c2:
+        text = f"{1 << 20}:0:{0x950}:hip_sixstep_512:1\n4096:0:{0x950}:hip_mixed:1\n"
+        A.check(A.lib().afft_wisdom_import(c2.h, text.encode()))
+        for n, plain_algo in ((1 << 20, "fourstep512x2048"), (4096, "mixed4096r")):
+            batch = 2 if n > 4096 else 16
+            x = synth.uniform_complex(n + 1, (batch, n), np.complex64)
+            ref = O.fft_batch(x.astype(np.complex128))
+            with A.Plan.from_desc(c2, (n,), batch=batch, flags=A.FLAG_MFMA) as p:
+                algo = p.describe()
+                y = np.empty_like(x)
+                p.forward(y, x)
+            assert "mfma" in algo, algo
+            assert rel_l2(y, ref) <= c64_tol(n)
+            with A.Plan.from_desc(c2, (n,), batch=batch) as p:
+                assert plain_algo in p.describe(), p.describe()
+        # strategies without an MFMA form are refused outright
+        for strat in (A.STRATEGY_MIXED, A.STRATEGY_BLUESTEIN):
+            with pytest.raises(A.ErrNotImplemented):
+                A.Plan.from_desc(c2, (4096,), flags=A.FLAG_MFMA, strategy=strat)

@@ -38,7 +38,7 @@ def test_ragged_three_forms_agree_with_oracle(ctx, dtype):
         xt = torch.from_numpy(x).to(dev)
         s = torch.cuda.Stream()
         outs = {}
-        for form in ("exec", "exec_graph", "exec_per_size"):
+        for form in ("exec", "exec_graph", "exec_per_size", "exec_per_job"):
             yt = torch.zeros_like(xt)
             with torch.cuda.stream(s):
                 getattr(rg, form)(A.FORWARD, yt.data_ptr(), xt.data_ptr(), s.cuda_stream)
