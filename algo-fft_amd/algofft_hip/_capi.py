@@ -62,6 +62,8 @@ SIGNATURES = [
     ("afft_plan_forward", c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_i64]),
     ("afft_plan_inverse", c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_i64]),
     ("afft_plan_exec_device", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("afft_plan_exec_device_noalloc", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("afft_noalloc_scratch_bytes", c_i64, []),
     ("afft_plan_forward_full", c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_i64]),
     ("afft_plan_inverse_full", c_int, [c_void_p, c_void_p, c_i64, c_void_p, c_i64]),
     ("afft_plan_exec_full_device", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),

@@ -153,6 +153,9 @@ int exec_plan(Plan& p, int dir, void* dst, const void* src, hipStream_t s);
 // The steps of direction di (0 forward, 1 inverse) with explicit workspace
 // buffers; allocates nothing and records no event.
 int launch_steps(Plan& p, int di, void* dst, const void* src, void* const* tmp, hipStream_t s);
+// Device address of the static scratch array of afft_plan_exec_device_noalloc
+// on `device`, looked up once per device (noalloc.hip).
+int noalloc_scratch(int device, void** out);
 
 // FP64 roots of unity W_L^m = exp(-2πi m / L), exact integer reduction of m.
 void root_of_unity(int64_t m, int64_t L, double* re, double* im);

@@ -205,6 +205,11 @@ int afft_context_create(int device_index, afft_context* out) {
         delete c;
         return set_error(AFFT_ERR_DEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
     }
+    // Set-up of the allocation-free entry point: the device address of its
+    // static scratch on this device.  A failure here is retried, and
+    // reported, by that entry point's first call.
+    void* scr = nullptr;
+    (void)noalloc_scratch(device_index, &scr);
     *out = c;
     clear_error();
     return AFFT_OK;

@@ -183,6 +183,17 @@ int afft_plan_inverse(afft_plan p, void* dst, int64_t dst_len, const void* src, 
  * wait for that execution first (outside stream capture), so executions of
  * one plan never overlap on its workspace; clones overlap freely. */
 int afft_plan_exec_device(afft_plan p, int direction, void* dst_dev, const void* src_dev, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they execute (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): afft_plan_exec_device with the
+ * plan's scratch taken from one static device array per GPU of
+ * afft_noalloc_scratch_bytes() bytes (64 MiB) instead of memory the plan
+ * allocates on its first execution.  Same kernels, same results.  A plan
+ * whose workspace exceeds the array fails with ErrInvalidLength before
+ * anything is enqueued.  All calls on one GPU share the array: calls that
+ * overlap in time on one GPU race on it (order them on one stream). */
+int afft_plan_exec_device_noalloc(afft_plan p, int direction, void* dst_dev, const void* src_dev, void* stream);
+int64_t afft_noalloc_scratch_bytes(void);
 /* PlanReal2D/3D.ForwardFull and InverseFull (plan_real_2d.go:231-276,
  * :370-389; plan_real_3d.go:241-285, :392-419) for a real plan of one
  * transform in the contiguous layout.  Forward: n reals in, the full
