@@ -92,6 +92,7 @@ SIGNATURES = [
     ("afft_tune_plan", c_int, [c_void_p, ctypes.POINTER(PlanDesc), c_int, c_int, ctypes.POINTER(c_int),
                                ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                c_int]),
+    ("afft_wisdom_features", ctypes.c_uint64, [c_void_p, ctypes.POINTER(PlanDesc)]),
     ("afft_wisdom_export", c_int, [c_void_p, ctypes.c_char_p, c_i64, ctypes.POINTER(c_i64)]),
     ("afft_wisdom_import", c_int, [c_void_p, ctypes.c_char_p]),
     ("afft_wisdom_clear", c_int, [c_void_p]),

@@ -87,6 +87,14 @@ struct Step {
     std::string name;
 };
 
+// Bluestein tables of one (n, padded m, filter layout), shared by a plan's
+// two directions: the FP64 chirp on the host and the device tables.
+struct BlueCache {
+    std::vector<double> ch;
+    const void* chirp = nullptr;
+    const void* filt[2] = {nullptr, nullptr};
+};
+
 struct Plan {
     Context* ctx = nullptr;
     hipStream_t stream = nullptr;
@@ -112,6 +120,7 @@ struct Plan {
     hipStream_t ws_stream = nullptr;
     hipEvent_t ws_done = nullptr;
     std::string algo;
+    std::map<std::tuple<int64_t, int64_t, int64_t>, BlueCache> blue_cache;
     std::mutex mu;
     ~Plan();
 };
@@ -157,8 +166,10 @@ int launch_steps(Plan& p, int di, void* dst, const void* src, void* const* tmp, 
 // on `device`, looked up once per device (noalloc.hip).
 int noalloc_scratch(int device, void** out);
 
-// FP64 roots of unity W_L^m = exp(-2πi m / L), exact integer reduction of m.
+// FP64 roots of unity W_L^m = exp(-2πi m / L), exact integer reduction of m
+// (and the same in long double, before the final rounding).
 void root_of_unity(int64_t m, int64_t L, double* re, double* im);
+void root_of_unity_ld(int64_t m, int64_t L, long double* re, long double* im);
 
 }  // namespace afft
 

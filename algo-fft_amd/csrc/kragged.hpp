@@ -2,8 +2,11 @@
 // 1-D complex transforms of different lengths (powers of two, 2/3/5/7-smooth
 // and Bluestein sizes) in one queue per task type.
 //
-// The batch is cut into tasks.  A job whose FFT length m fits in one block
-// (m <= RAG_WHOLE) is one task.  A larger job is split four-step style,
+// The batch is cut into tasks.  Jobs whose FFT length m fits in one block
+// (m <= rag_whole_cap: 16384 points complex64, 8192 complex128) run whole:
+// power-of-two lengths from 2048 and Bluestein paddings from 2048 on the
+// compile-time cores (one launch per length), every other length in one
+// launch of the runtime passes.  A larger job is split four-step style,
 // m = m1 * m2, into tasks that each hold a few whole sub-transforms in LDS:
 //
 //   column task   G adjacent columns n2 of x[m2*n1 + n2]: FFT over n1, times
@@ -38,8 +41,21 @@ namespace afft {
 constexpr int RAG_THREADS = 256;
 constexpr int RAG_PTS = 4096;       // points per block (whole job, or G*L of a task)
 constexpr int RAG_BUF = 4352;       // LDS buffer in points (G*(L+1) <= 4096 + G)
-constexpr int RAG_WHOLE = 4096;     // largest FFT length run as a single task
-constexpr int RAG_SPASS = 12;       // passes of a schedule of length <= 4096
+constexpr int RAG_WHOLE = 4096;     // largest sub-transform length of a split task
+constexpr int RAG_SPASS = 12;       // passes of a schedule of length <= 16384
+
+// Largest FFT length run as one task: the whole transform in 128 KiB of LDS.
+constexpr int rag_whole_cap(int prec) { return prec ? 8192 : 16384; }
+
+// Runtime-pass whole jobs: blocks of PTS points (the largest whole job) and
+// PTS/16 threads; a task holds G = PTS/L jobs of one length L (a power of
+// two), so every task moves about the same data and one launch serves every
+// length.
+template <typename T>
+struct RagW {
+    static constexpr int PTS = sizeof(T) == 4 ? 16384 : 8192;
+    static constexpr int THREADS = PTS / 16;
+};
 
 struct RSched {
     int npass;
@@ -250,13 +266,14 @@ struct ColLanes {
 
 // Flat index i = tid + s * RAG_THREADS of rows of length L, as (row c,
 // element n), advanced by one step without a division.
+template <int TH = RAG_THREADS>
 struct RowWalk {
     int c, n, dq, dr, L;
     __device__ explicit RowWalk(int len) : L(len) {
         c = (int)threadIdx.x / L;
         n = (int)threadIdx.x - c * L;
-        dq = RAG_THREADS / L;
-        dr = RAG_THREADS - dq * L;
+        dq = TH / L;
+        dr = TH - dq * L;
     }
     __device__ void next() {
         n += dr;
@@ -268,14 +285,15 @@ struct RowWalk {
     }
 };
 
-__device__ __forceinline__ int flat_steps(int total) { return (total + RAG_THREADS - 1) / RAG_THREADS; }
+template <int TH = RAG_THREADS>
+__device__ __forceinline__ int flat_steps(int total) { return (total + TH - 1) / TH; }
 
 // Contiguous run src[0 .. total) into LDS rows of pitch L + 1 (row c of the
 // run starts at src + c*L).
 template <typename T>
 __device__ __forceinline__ void rag_stage_rows(const cx<T>* __restrict__ src, int total, int L, cx<T>* lds) {
     const int tid = threadIdx.x;
-    RowWalk w(L);
+    RowWalk<> w(L);
     rag_staged<sizeof(cx<T>)>(
         flat_steps(total), [&](int s) { return src[min(tid + s * RAG_THREADS, total - 1)]; },
         [&](int s, cx<T> v) {
@@ -288,25 +306,26 @@ __device__ __forceinline__ void rag_stage_rows(const cx<T>* __restrict__ src, in
 
 // Whole jobs of one length L on the runtime passes: a task holds up to
 // G = jb.g_col of them (ids at a.joblist[first .. first+count)), one LDS row
-// of pitch L + 1 and RAG_THREADS/G threads each.  The inputs are staged with
-// contiguous lanes along each row; the last pass writes each job's output.
-template <typename T, int DIR>
+// of pitch L + 1 and TH/G threads each (TH threads per block, at most 16
+// points per thread).  The inputs are staged with contiguous lanes along
+// each row; the last pass writes each job's output.
+template <typename T, int DIR, int TH = RAG_THREADS>
 __device__ void rag_whole(const RagArgs a, const RJob& j0, const RTask& tk, cx<T>* lds, long long* base) {
-    const int L = j0.m, P = L + 1, TPT = RAG_THREADS / j0.g_col, cnt = tk.count, tid = threadIdx.x;
+    const int L = j0.m, P = L + 1, TPT = TH / j0.g_col, cnt = tk.count, tid = threadIdx.x;
     const int total = cnt * L;
     if (tid < cnt) base[tid] = a.jobs[a.joblist[tk.first + tid]].in_off;
     __syncthreads();
     const cx<T>* __restrict__ x = (const cx<T>*)a.in;
-    RowWalk rl(L), rw(L);
+    RowWalk<TH> rl(L), rw(L);
     rag_staged<sizeof(cx<T>)>(
-        flat_steps(total),
+        flat_steps<TH>(total),
         [&](int) {
             const cx<T> v = x[base[min(rl.c, cnt - 1)] + rl.n];
             rl.next();
             return v;
         },
         [&](int s, cx<T> v) {
-            const int i = tid + s * RAG_THREADS;
+            const int i = tid + s * TH;
             if (i < total) lds[i + rw.c] = v;
             rw.next();
         });
@@ -318,8 +337,10 @@ __device__ void rag_whole(const RagArgs a, const RJob& j0, const RTask& tk, cx<T
     rag_passes<T, DIR>(j0.s[0], L, P, cnt, TPT, lds, sink);
 }
 
-// A whole Bluestein job: pre-chirp and zero padding on the staging load,
-// forward FFT, filter product, inverse FFT, post-chirp on the store.
+// A whole Bluestein job on the runtime passes (one job per block of
+// RAG_THREADS threads, m <= RAG_PTS): pre-chirp and zero padding on the
+// staging load, forward FFT, filter product, inverse FFT, post-chirp on the
+// store.
 template <typename T>
 __device__ void rag_whole_blue(const RagArgs a, const RJob& jb, cx<T>* lds) {
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
@@ -412,9 +433,10 @@ __device__ void rag_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>
 }
 
 // ---- power-of-two sub-transforms on the compile-time cores -------------------
-// Split jobs whose sub-transform lengths are powers of two (all plain
-// power-of-two jobs above RAG_WHOLE, and every Bluestein job, whose m is its
-// power-of-two padding; m <= 2^21 gives m1 in [64, 1024], m2 in [128, 2048])
+// Split jobs whose sub-transform lengths are powers of two (plain
+// power-of-two jobs above the whole-job cap, and every split Bluestein job,
+// whose m is its power-of-two padding; m <= 2^21 gives m1 in [64, 1024], m2
+// in [128, 2048])
 // run their column and row tasks on the radix-16 register cores of the
 // single-size kernels: 16 points per thread, L/16 threads per sub-transform,
 // G = RAG_PTS/L sub-transforms per task (the host cuts every such job into
@@ -435,6 +457,8 @@ template <> struct P2Sched<512> { using S = Sched<2, 16, 16>; };
 template <> struct P2Sched<1024> { using S = Sched<4, 16, 16>; };
 template <> struct P2Sched<2048> { using S = Sched<8, 16, 16>; };
 template <> struct P2Sched<4096> { using S = Sched<16, 16, 16>; };
+template <> struct P2Sched<8192> { using S = Sched<2, 16, 16, 16>; };
+template <> struct P2Sched<16384> { using S = Sched<4, 16, 16, 16>; };
 
 // LDS of the row tasks: G padded slabs (the column tasks need RAG_PTS).
 template <typename T>
@@ -541,14 +565,18 @@ __device__ __forceinline__ void p2_rows_blue(const RagArgs a, const RJob& jb, co
     }
 }
 
-// Whole power-of-two jobs of one length L: a task holds up to G = RAG_PTS/L
-// of them (their ids at a.joblist[first .. first+count)), L/16 threads each,
-// loaded straight into registers and stored in natural order.  The table
-// pointer is the first job's: every job of the task has the same schedule.
+// Whole power-of-two jobs of one length L: a task holds up to
+// G = max(1, RAG_PTS/L) of them (their ids at a.joblist[first .. first+count)),
+// L/16 threads each, loaded straight into registers and stored in natural
+// order.  Lengths above RAG_PTS run one job per block of L/16 threads.  The
+// table pointer is the first job's: every job of the task has the same
+// schedule.
 template <typename T, int L>
 struct P2Whole {
     static constexpr int PADSH = PadFor<T, L>::value;
-    static constexpr int LDS = RAG_PTS / L * LdsRow<L, PADSH>::SLAB;
+    static constexpr int G = L >= RAG_PTS ? 1 : RAG_PTS / L;
+    static constexpr int THREADS = G * L / 16;
+    static constexpr int LDS = G * LdsRow<L, PADSH>::SLAB;
 };
 
 template <typename T, int L, int DIR>
@@ -569,6 +597,56 @@ __device__ __forceinline__ void p2_whole(const RagArgs a, const RTask& tk, cx<T>
         cx<T>* __restrict__ y = (cx<T>*)a.out + a.jobs[jid].out_off;
 #pragma unroll
         for (int e = 0; e < E; ++e) y[t + TT * e] = scl(v[e], sc);
+    }
+}
+
+// Whole Bluestein jobs of one padded length L (grouped like p2_whole; the
+// jobs of a task differ in n, so each group reads its own job's chirp and
+// filter): pre-chirp and zero padding on the load, forward FFT, filter
+// product, inverse FFT, post-chirp (and 1/n for the inverse transform) on
+// the store, all on the compile-time core.
+// Points per thread of the Bluestein whole-job cores (16: L/16 threads per
+// job; 32 points a thread at 16384 points measured worse at compile time,
+// 256 registers and more spills than the 128-register 1024-thread form).
+template <typename T, int L>
+struct P2WholeBlue {
+    static constexpr int E = 16;
+    static constexpr int THREADS = P2Whole<T, L>::G * L / E;
+};
+
+template <typename T, int L>
+__device__ __forceinline__ void p2_whole_blue(const RagArgs a, const RTask& tk, cx<T>* lds) {
+    using S = typename P2Sched<L>::S;
+    constexpr int E = P2WholeBlue<T, L>::E, TT = L / E;
+    const int tid = threadIdx.x, c = tid / TT, t = tid % TT;
+    const bool act = c < tk.count;
+    const RJob& jb = a.jobs[a.joblist[tk.first + (act ? c : 0)]];
+    const bool inv = a.dir > 0;
+    const int n = jb.n;
+    const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
+    const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
+    const cx<T>* __restrict__ filt = (const cx<T>*)jb.filt[inv ? 1 : 0];
+    const cx<T>* __restrict__ tw = (const cx<T>*)a.jobs[a.joblist[tk.first]].s[0].tw;
+    const LdsRow<L, P2Whole<T, L>::PADSH> li(c);
+    cx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = t + TT * e;
+        v[e] = mk<T>(0, 0);
+        if (i < n) v[e] = inv ? cmulc(x[i], chirp[i]) : cmul(x[i], chirp[i]);
+    }
+    core<T, S, E, -1>(v, lds, t, tw, li);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = cmul(v[e], filt[t + TT * e]);
+    core<T, S, E, 1, true>(v, lds, t, tw, li);
+    if (act) {
+        const T sc = inv ? T(1) / T(n) : T(1);
+        cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = t + TT * e;
+            if (i < n) y[i] = scl(inv ? cmulc(v[e], chirp[i]) : cmul(v[e], chirp[i]), sc);
+        }
     }
 }
 
@@ -650,23 +728,36 @@ __global__ void __launch_bounds__(RAG_THREADS) k_rag_cols2(RagArgs a) {
     rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_cols2<T, L>(a, jb, tk, lds); });
 }
 
+// The whole-job kernels.  One kernel per length L for the compile-time
+// cores: a kernel holding several lengths needs the registers of all of
+// them (and spills at 1024 threads per block), so lengths below 2048, whose
+// jobs hold little of the data, run on the runtime passes instead, all
+// lengths in one launch.
 template <typename T, int L, int DIR>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_whole_p2(RagArgs a) {
+__global__ void __launch_bounds__((P2Whole<T, L>::THREADS)) k_rag_whole_p2(RagArgs a) {
     __shared__ cx<T> lds[P2Whole<T, L>::LDS];
     rag_queue(a, [=](const RJob&, const RTask& tk) { p2_whole<T, L, DIR>(a, tk, lds); });
 }
 
-template <typename T, int DIR>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_whole(RagArgs a) {
-    __shared__ cx<T> lds[RAG_BUF];
-    __shared__ long long base[RAG_THREADS];
-    rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_whole<T, DIR>(a, jb, tk, lds, base); });
+template <typename T, int L>
+__global__ void __launch_bounds__((P2WholeBlue<T, L>::THREADS)) k_rag_whole_blue_p2(RagArgs a) {
+    __shared__ cx<T> lds[P2Whole<T, L>::LDS];
+    rag_queue(a, [=](const RJob&, const RTask& tk) { p2_whole_blue<T, L>(a, tk, lds); });
 }
 
 template <typename T>
 __global__ void __launch_bounds__(RAG_THREADS) k_rag_whole_blue(RagArgs a) {
     __shared__ cx<T> lds[RAG_BUF];
     rag_queue(a, [=](const RJob& jb, const RTask&) { rag_whole_blue<T>(a, jb, lds); });
+}
+
+template <typename T, int DIR>
+__global__ void __launch_bounds__(RagW<T>::THREADS) k_rag_whole(RagArgs a) {
+    __shared__ cx<T> lds[RagW<T>::PTS + RagW<T>::THREADS];
+    __shared__ long long base[RagW<T>::THREADS];
+    rag_queue(a, [=](const RJob& jb, const RTask& tk) {
+        rag_whole<T, DIR, RagW<T>::THREADS>(a, jb, tk, lds, base);
+    });
 }
 
 

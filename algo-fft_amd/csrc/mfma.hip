@@ -13,20 +13,28 @@
 //                                        product, 8+2q, 9+2q of the second)
 // FP16 keeps 11 significant bits, so data and matrix are split into hi + lo
 // halves and three products hi*hi + hi*lo + lo*hi accumulate in FP32
-// (about 22 bits).  Before the split each butterfly is scaled by a power of
-// two so its largest component lies in [0.5, 1): FP16 never overflows, and
-// the scale is undone exactly on the outputs.  Twiddles are applied in FP32
-// before the split.
+// (about 22 bits).  Twiddles are applied in FP32 before the split.
 //
-// Lengths 256 (16x16), 1024 (4x16x16, the radix-4 pass on the vector ALUs)
-// and 4096 (16x16x16).  A block holds G transforms in one LDS buffer (rows of
-// pitch N + N/16, one pad slot per 16 points) and has one thread per 16
-// points; each wave runs four 16-butterfly groups per pass, reading all of
-// them before a barrier and writing after it, so the passes work in place.
-// Rows (VAR_ROW, and the input of VAR_4B) are read and written straight from
-// global memory, whose index runs along the butterflies; the four-step column
-// pass (VAR_4A) and the transposed store of VAR_4B go through LDS staging
-// loops whose lanes run along the transform index.
+// Scaling.  Each transform is scaled once, by a power of two 2^-e chosen so
+// that its largest input component lies in [0.5, 1); the scale is undone
+// exactly on the final store.  A radix-R pass grows the largest complex
+// modulus by at most R, so the inputs of the MFMA passes of a transform of
+// up to 16384 points stay below sqrt(2) * 4096 < 65504 and FP16 never
+// overflows; the hi/lo split keeps 22 bits down to components of 2^-14 of
+// the transform's largest input and an absolute 2^-25 of it below that
+// (FP16 subnormals).  The per-transform maximum is one wave reduction per
+// transform and wave plus an LDS atomic, once per kernel.
+//
+// Lengths 256 .. 16384: radix-16 passes on the matrix cores, with a first
+// radix-2/4/8 pass on the vector ALUs when the length is not a power of 16.
+// A block holds G transforms in one LDS buffer (rows of pitch N + N/16, one
+// pad slot per 16 points) and has one thread per 16 points; each wave runs
+// four 16-butterfly groups per pass, reading all of them before a barrier and
+// writing after it, so the passes work in place.  Rows (VAR_ROW, and the
+// input of VAR_4B) are read and written straight from global memory, whose
+// index runs along the butterflies; the four-step column pass (VAR_4A) and
+// the transposed store of VAR_4B go through LDS staging loops whose lanes run
+// along the transform index.
 #include "afft_internal.hpp"
 
 namespace afft {
@@ -34,8 +42,13 @@ namespace afft {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int mf_rad(int N, int p) { return (N == 1024 && p == 0) ? 4 : 16; }
-constexpr int mf_npass(int N) { return N == 256 ? 2 : 3; }
+constexpr int mf_lg(int N) { return N <= 1 ? 0 : 1 + mf_lg(N / 2); }
+// remainder radix of the first pass (1: none, the length is a power of 16)
+constexpr int mf_r0(int N) { return 1 << (mf_lg(N) % 4); }
+constexpr int mf_npass(int N) { return mf_lg(N) / 4 + (mf_r0(N) > 1 ? 1 : 0); }
+constexpr int mf_rad(int N, int p) { return (p == 0 && mf_r0(N) > 1) ? mf_r0(N) : 16; }
+// the first pass on the matrix cores
+constexpr int mf_p0(int N) { return mf_r0(N) > 1 ? 1 : 0; }
 constexpr int mf_ns(int N, int p) {
     int ns = 1;
     for (int i = 0; i < p; ++i) ns *= mf_rad(N, i);
@@ -62,28 +75,51 @@ __device__ __forceinline__ float dft16_coef(int o, int kk) {
 // 2^e as a float, e in [-126, 127]
 __device__ __forceinline__ float pow2f(int e) { return __int_as_float((127 + e) << 23); }
 
-// One radix-16 butterfly per lane column: x[i] holds input r = 4q + i; y gets
-// outputs 2q, 2q+1, 8+2q, 9+2q.
+// Exponent e of the transform scale 2^-e from the bits of its largest input
+// component m (m = f * 2^e with f in [0.5, 1)), clamped so that both 2^-e
+// and 2^e are normal floats.
+__device__ __forceinline__ int mf_exp(unsigned mbits) {
+    int e = (int)((mbits >> 23) & 0xff) - 126;
+    return e > 126 ? 126 : (e < -126 ? -126 : e);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+
+// Maximum over the lanes l ^ x of every xor mask x >= XMIN (a power of two):
+// the whole wave for XMIN = 1.  Within rows of 16 lanes by DPP row
+// rotations (after rotating by 8, 4, 2, 1 lane i holds the maximum over the
+// lanes i + k*XMIN of its row), across rows by lane shuffles.
+template <int XMIN>
+__device__ __forceinline__ float wave_max(float v) {
+    if constexpr (XMIN <= 8) v = fmaxf(v, dpp_f<0x128>(v));   // row_ror:8
+    if constexpr (XMIN <= 4) v = fmaxf(v, dpp_f<0x124>(v));   // row_ror:4
+    if constexpr (XMIN <= 2) v = fmaxf(v, dpp_f<0x122>(v));   // row_ror:2
+    if constexpr (XMIN <= 1) v = fmaxf(v, dpp_f<0x121>(v));   // row_ror:1
+    if constexpr (XMIN <= 16) v = fmaxf(v, __shfl_xor(v, 16));
+    v = fmaxf(v, __shfl_xor(v, 32));
+    return v;
+}
+
+__device__ __forceinline__ float cmax(cx<float> v, float m) { return fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y))); }
+
+// One radix-16 butterfly per lane column: x[i] holds input r = 4q + i
+// (already scaled); y gets outputs 2q, 2q+1, 8+2q, 9+2q.  y may be x: every
+// input is read before the first output is written.
 __device__ __forceinline__ void mf_dft16(const half8 (&Ah)[2], const half8 (&Al)[2], const cx<float> (&x)[4],
                                          cx<float> (&y)[4]) {
-    float m = 0.f;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) m = fmaxf(m, fmaxf(fabsf(x[i].x), fabsf(x[i].y)));
-    m = fmaxf(m, __shfl_xor(m, 16));
-    m = fmaxf(m, __shfl_xor(m, 32));
-    // m = f * 2^e with f in [0.5, 1); e clamped so both scale factors exist
-    int e = ((__float_as_int(m) >> 23) & 0xff) - 126;
-    e = e > 126 ? 126 : e;
-    const float down = pow2f(-e), up = pow2f(e);
     half8 bh, bl;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float re = x[i].x * down, im = x[i].y * down;
+        const float re = x[i].x, im = x[i].y;
         const _Float16 hr = (_Float16)re, hi = (_Float16)im;
         bh[2 * i] = hr;
         bh[2 * i + 1] = hi;
-        bl[2 * i] = (_Float16)(re - (float)hr);
-        bl[2 * i + 1] = (_Float16)(im - (float)hi);
+        // lo = x - hi, exact in FP32 and rounded once (one mixed-precision FMA)
+        bl[2 * i] = (_Float16)__builtin_fmaf(re, 1.0f, -(float)hr);
+        bl[2 * i + 1] = (_Float16)__builtin_fmaf(im, 1.0f, -(float)hi);
     }
     f32x4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
     d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[0], bh, d0, 0, 0, 0);
@@ -92,10 +128,10 @@ __device__ __forceinline__ void mf_dft16(const half8 (&Ah)[2], const half8 (&Al)
     d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[1], bh, d1, 0, 0, 0);
     d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ah[1], bl, d1, 0, 0, 0);
     d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Al[1], bh, d1, 0, 0, 0);
-    y[0] = mk<float>(d0[0] * up, d0[1] * up);
-    y[1] = mk<float>(d0[2] * up, d0[3] * up);
-    y[2] = mk<float>(d1[0] * up, d1[1] * up);
-    y[3] = mk<float>(d1[2] * up, d1[3] * up);
+    y[0] = mk<float>(d0[0], d0[1]);
+    y[1] = mk<float>(d0[2], d0[3]);
+    y[2] = mk<float>(d1[0], d1[1]);
+    y[3] = mk<float>(d1[2], d1[3]);
 }
 
 template <int N, int G>
@@ -106,12 +142,15 @@ struct MfCfg {
     static constexpr int GROUPS = NB / 16;        // 16-butterfly groups per transform
     static constexpr int PITCH = N + N / 16;
     static constexpr int P = mf_npass(N);
+    static constexpr int P0 = mf_p0(N);
     static_assert(G * GROUPS == 4 * WAVES, "four groups per wave");
+    static_assert(T % 64 == 0 && T <= 1024, "whole waves, at most 1024 threads");
 };
 
 template <int N, int G, int VAR, int DIR, int p>
-__device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, const half8 (&Ah)[2], const half8 (&Al)[2],
-                                        const cx<float>* __restrict__ in, cx<float>* __restrict__ out, long long c0) {
+__device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, unsigned* smax, const half8 (&Ah)[2],
+                                        const half8 (&Al)[2], const cx<float>* __restrict__ in,
+                                        cx<float>* __restrict__ out, long long c0) {
     using C = MfCfg<N, G>;
     constexpr int R = mf_rad(N, p), NS = mf_ns(N, p), TWO = mf_twoff(N, p);
     constexpr bool GIN = p == 0 && VAR != VAR_4A;          // first pass reads global rows
@@ -120,7 +159,7 @@ __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, const ha
     const int q = lane >> 4, col = lane & 15;
     const cx<float>* __restrict__ tw = (const cx<float>*)a.tw;
     if constexpr (R == 16) {
-        cx<float> y[4][4];
+        cx<float> x[4][4];   // inputs, then (in place) outputs of the four items
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
             const int item = wave + C::WAVES * it;
@@ -128,18 +167,52 @@ __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, const ha
             const int jb = 16 * g + col;
             const long long cg = c0 + c;
             const bool act = cg < a.count;
-            cx<float> x[4];
+            // rows past the batch read the last row (no branch) and are zeroed
+            const long long cl = act ? cg : a.count - 1;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int n = jb + (4 * q + i) * C::NB;
+                if constexpr (GIN) {
+                    const cx<float> v = in[cl * a.in_cd + n];
+                    x[it][i] = act ? v : mk<float>(0, 0);
+                } else {
+                    x[it][i] = lds[c * C::PITCH + mf_pad(n)];
+                }
+            }
+        }
+        if constexpr (GIN) {
+            // this pass reads the input: the per-transform maximum (every
+            // lane of an item belongs to the item's transform)
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                float m = 0.f;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) m = cmax(x[it][i], m);
+                m = wave_max<1>(m);
+                const int item = wave + C::WAVES * it;
+                if (lane == 0) atomicMax(&smax[item / C::GROUPS], __float_as_uint(m));
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int item = wave + C::WAVES * it;
+            const int c = item / C::GROUPS, g = item - c * C::GROUPS;
+            const int jb = 16 * g + col;
+            float down = 1.f;
+            if constexpr (p == C::P0) down = pow2f(-mf_exp(smax[c]));
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = 4 * q + i;
-                const int n = jb + r * C::NB;
-                if constexpr (GIN) x[i] = act ? in[cg * a.in_cd + n] : mk<float>(0, 0);
-                else x[i] = lds[c * C::PITCH + mf_pad(n)];
                 if constexpr (NS > 1) {
-                    if (r > 0) x[i] = twmul<DIR>(x[i], tw[TWO + (r - 1) * NS + jb % NS]);   // r-major table
+                    // r-major table; r = 0 (q = 0, i = 0) takes entry r = 1's
+                    // slot and discards it, so no lane branches around a load
+                    const cx<float> w = tw[TWO + (r > 0 ? r - 1 : 0) * NS + jb % NS];
+                    x[it][i] = r > 0 ? twmul<DIR>(x[it][i], w) : x[it][i];
                 }
+                if constexpr (p == C::P0) x[it][i] = scl(x[it][i], down);
             }
-            mf_dft16(Ah, Al, x, y[it]);
+            mf_dft16(Ah, Al, x[it], x[it]);
         }
         if constexpr (!GIN) __syncthreads();   // every read of this pass precedes any write
 #pragma unroll
@@ -150,62 +223,77 @@ __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, const ha
             const int base = (jb / NS) * NS * 16 + jb % NS;
             const int ko[4] = {2 * q, 2 * q + 1, 8 + 2 * q, 9 + 2 * q};
             const long long cg = c0 + c;
+            float sc = 1.f;
+            if constexpr (GOUT) sc = (float)a.scale * pow2f(mf_exp(smax[c]));
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int idx = base + ko[i] * NS;
                 if constexpr (GOUT) {
-                    if (cg < a.count) st_out(out + cg * a.out_cd + idx, scl(y[it][i], (float)a.scale), a.nt);
+                    if (cg < a.count) st_out(out + cg * a.out_cd + idx, scl(x[it][i], sc), a.nt);
                 } else {
-                    lds[c * C::PITCH + mf_pad(idx)] = y[it][i];
+                    lds[c * C::PITCH + mf_pad(idx)] = x[it][i];
                 }
             }
         }
     } else {
-        // radix 4 on the vector ALUs (first pass of N = 1024, Ns = 1):
-        // butterflies b = tid + T*i of the block, transform b / 256
-        static_assert(R == 4 && p == 0 && NS == 1, "radix 4 only as the first pass");
-        cx<float> v[4][4];
+        // radix 2, 4 or 8 on the vector ALUs (first pass, Ns = 1):
+        // butterflies b = tid + T*i of the block, transform b / (N/R)
+        static_assert(p == 0 && NS == 1 && 16 % R == 0, "small radix only as the first pass");
+        constexpr int NBUT = 16 / R;
+        cx<float> v[NBUT][R];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < NBUT; ++i) {
             const int b = tid + C::T * i;
-            const int c = b / (N / 4), j = b - c * (N / 4);
+            const int c = b / (N / R), j = b - c * (N / R);
             const long long cg = c0 + c;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int n = j + r * (N / 4);
+            for (int r = 0; r < R; ++r) {
+                const int n = j + r * (N / R);
                 if constexpr (GIN) v[i][r] = cg < a.count ? in[cg * a.in_cd + n] : mk<float>(0, 0);
                 else v[i][r] = lds[c * C::PITCH + mf_pad(n)];
             }
-            dft<4, DIR, 1>(v[i]);
+            if constexpr (GIN) {
+                // the per-transform maximum of the input (a wave's butterflies
+                // i belong to one transform: N/R is a multiple of 64)
+                float m = 0.f;
+#pragma unroll
+                for (int r = 0; r < R; ++r) m = cmax(v[i][r], m);
+                m = wave_max<1>(m);
+                if (lane == 0) atomicMax(&smax[c], __float_as_uint(m));
+            }
+            dft<R, DIR, 1>(v[i]);
         }
         if constexpr (!GIN) __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < NBUT; ++i) {
             const int b = tid + C::T * i;
-            const int c = b / (N / 4), j = b - c * (N / 4);
+            const int c = b / (N / R), j = b - c * (N / R);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) lds[c * C::PITCH + mf_pad(4 * j + r)] = v[i][r];
+            for (int r = 0; r < R; ++r) lds[c * C::PITCH + mf_pad(R * j + r)] = v[i][r];
         }
     }
     if constexpr (!GOUT) __syncthreads();
 }
 
 template <int N, int G, int VAR, int DIR, int... Ps>
-__device__ __forceinline__ void mf_passes(const KArgs& a, cx<float>* lds, const half8 (&Ah)[2], const half8 (&Al)[2],
-                                          const cx<float>* in, cx<float>* out, long long c0,
+__device__ __forceinline__ void mf_passes(const KArgs& a, cx<float>* lds, unsigned* smax, const half8 (&Ah)[2],
+                                          const half8 (&Al)[2], const cx<float>* in, cx<float>* out, long long c0,
                                           std::integer_sequence<int, Ps...>) {
-    (mf_pass<N, G, VAR, DIR, Ps>(a, lds, Ah, Al, in, out, c0), ...);
+    (mf_pass<N, G, VAR, DIR, Ps>(a, lds, smax, Ah, Al, in, out, c0), ...);
 }
 
 template <int N, int G, int VAR, int DIR>
 __global__ void __launch_bounds__((MfCfg<N, G>::T)) k_mfma(KArgs a) {
     using C = MfCfg<N, G>;
     __shared__ cx<float> lds[G * C::PITCH];
-    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ unsigned smax[G];   // per-transform largest input component (float bits)
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const int q = lane >> 4, col = lane & 15;
     const long long c0 = (long long)blockIdx.x * G;
     const cx<float>* __restrict__ in = (const cx<float>*)a.in + (long long)blockIdx.y * a.in_bd;
     cx<float>* __restrict__ out = (cx<float>*)a.out + (long long)blockIdx.y * a.out_bd;
+    if (tid < G) smax[tid] = 0u;
 
     // A operands (fixed): [product][hi/lo]
     half8 Ah[2], Al[2];
@@ -218,23 +306,31 @@ __global__ void __launch_bounds__((MfCfg<N, G>::T)) k_mfma(KArgs a) {
             Ah[pr][e] = h;
             Al[pr][e] = (_Float16)(v - (float)h);
         }
+    __syncthreads();   // smax cleared
 
     if constexpr (VAR == VAR_4A) {
-        // columns: lanes run along the transform index c
+        // columns: lanes run along the transform index c = tid mod G, which
+        // is the same in every step
+        static_assert(C::T % G == 0, "one column per thread");
+        float m = 0.f;
         for (int i = tid; i < G * N; i += C::T) {
             const int n = i / G, c = i - n * G;
             const long long cg = c0 + c;
-            lds[c * C::PITCH + mf_pad(n)] = cg < a.count ? in[cg * a.in_cd + (long long)n * a.in_nd] : mk<float>(0, 0);
+            const cx<float> v = cg < a.count ? in[cg * a.in_cd + (long long)n * a.in_nd] : mk<float>(0, 0);
+            m = cmax(v, m);
+            lds[c * C::PITCH + mf_pad(n)] = v;
         }
+        m = wave_max<G>(m);   // over the lanes of this column
+        if (lane < G) atomicMax(&smax[lane], __float_as_uint(m));
         __syncthreads();
     }
-    mf_passes<N, G, VAR, DIR>(a, lds, Ah, Al, in, out, c0, std::make_integer_sequence<int, C::P>{});
+    mf_passes<N, G, VAR, DIR>(a, lds, smax, Ah, Al, in, out, c0, std::make_integer_sequence<int, C::P>{});
     if constexpr (VAR == VAR_4A || VAR == VAR_4B) {
         // thread tid stores points k = tid/G + s*(T/G) of column c = tid mod G
         constexpr int STEPS = G * N / C::T, KS = C::T / G;
         static_assert(C::T % G == 0 && (G * N) % C::T == 0, "whole columns per step");
-        const float sc = (float)a.scale;
         const int c = tid % G, k0 = tid / G;
+        const float sc = (float)a.scale * pow2f(mf_exp(smax[c]));
         const long long cg = c0 + c;
         const long long col = cg + a.tw4_c0;
         // W_L^{k*col}: a table value every fourth step, the step factor
@@ -255,7 +351,9 @@ __global__ void __launch_bounds__((MfCfg<N, G>::T)) k_mfma(KArgs a) {
 }
 
 // Kernel, transforms per block and threads per block of the MFMA form for a
-// length and layout; nullptr when the form does not cover them.
+// length and layout; nullptr when the form does not cover them.  Rows take
+// G = max(1, 4096/N) transforms per block; the four-step passes G columns
+// within 1024 threads and 128 KiB of LDS (16 up to 1024 points, then 8 and 4).
 const void* mfma_kernel(int n, int var, int dir, int* per_block, int* threads) {
 #define AFFT_MF(NN, GG, VV)                                                                  \
     if (n == NN && var == VV) {                                                              \
@@ -264,12 +362,22 @@ const void* mfma_kernel(int n, int var, int dir, int* per_block, int* threads) {
         return dir < 0 ? (const void*)&k_mfma<NN, GG, VV, -1> : (const void*)&k_mfma<NN, GG, VV, 1>; \
     }
     AFFT_MF(256, 16, VAR_ROW)
+    AFFT_MF(512, 8, VAR_ROW)
     AFFT_MF(1024, 4, VAR_ROW)
+    AFFT_MF(2048, 2, VAR_ROW)
     AFFT_MF(4096, 1, VAR_ROW)
+    AFFT_MF(8192, 1, VAR_ROW)
+    AFFT_MF(16384, 1, VAR_ROW)
     AFFT_MF(256, 16, VAR_4A)
+    AFFT_MF(512, 16, VAR_4A)
     AFFT_MF(1024, 16, VAR_4A)
+    AFFT_MF(2048, 8, VAR_4A)
+    AFFT_MF(4096, 4, VAR_4A)
     AFFT_MF(256, 16, VAR_4B)
+    AFFT_MF(512, 16, VAR_4B)
     AFFT_MF(1024, 16, VAR_4B)
+    AFFT_MF(2048, 8, VAR_4B)
+    AFFT_MF(4096, 4, VAR_4B)
 #undef AFFT_MF
     return nullptr;
 }

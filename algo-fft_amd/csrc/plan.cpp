@@ -142,10 +142,10 @@ struct AxisJob {
 static int emit_axis(Plan& p, int di, const AxisJob& j, std::string& algo);
 
 
-// MFMA form (AFFT_FLAG_MFMA): complex64 rows of N = 256 or 4096.
+// MFMA form (AFFT_FLAG_MFMA): complex64 rows of N = 256 .. 16384 (mfma.hip).
 const void* mfma_kernel(int n, int var, int dir, int* per_block, int* threads);
 
-// MFMA form of a single-pass row transform (n = 256, 1024 or 4096).
+// MFMA form of a single-pass row transform (n = 256 .. 16384).
 static int emit_mfma(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int dir = di ? 1 : -1;
     int g = 1, threads = 256;
@@ -153,7 +153,7 @@ static int emit_mfma(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const KernelInfo* k = find_kernel(0, (int)j.n, VAR_ROW, dir);
     if (p.desc.precision != 0 || !fn || !k || j.in_nd != 1 || j.out_nd != 1)
         return set_error(AFFT_ERR_NOT_IMPLEMENTED,
-                         "the MFMA form covers complex64 rows of 256, 1024, 4096, 65536, 262144 or 1048576 points");
+                         "the MFMA form covers complex64 rows of 2^8 .. 2^24 points");
     KArgs a{};
     int rc = schedule_twiddles(p, k, &a.tw);
     if (rc) return rc;
@@ -344,13 +344,12 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const bool mf = p.desc.flags & AFFT_FLAG_MFMA;
     int64_t n1 = fourstep_split(prec, lg);
     if (mf) {
-        // the MFMA passes have lengths 256 and 1024
-        n1 = 0;
-        for (int64_t c1 : {256, 1024})
-            if (!n1 && (j.n == c1 * 256 || j.n == c1 * 1024) && (!j.split || j.split == c1)) n1 = c1;
-        if (!n1 || prec != 0)
+        // both MFMA passes have lengths in [256, 4096]: the balanced split
+        // (n1 <= n2) unless a split is forced
+        n1 = j.split ? j.split : (int64_t)1 << (lg / 2);
+        if (!is_pow2(n1) || n1 < 256 || n1 > 4096 || j.n / n1 < 256 || j.n / n1 > 4096 || prec != 0)
             return set_error(AFFT_ERR_NOT_IMPLEMENTED,
-                             "the MFMA form covers complex64 rows of 256, 1024, 4096, 65536, 262144 or 1048576 points");
+                             "the MFMA form covers complex64 rows of 2^8 .. 2^24 points");
     } else if (j.split) {
         if (!is_pow2(j.split) || j.split < 2 || j.split >= j.n)
             return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step split must be a power of two below n");
@@ -608,7 +607,9 @@ static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& alg
     for (int64_t d = 1; d * d <= j.n; ++d)
         if (j.n % d == 0 && d <= cap && j.n / d <= cap) n1 = d;
     if (j.split) {
-        if (j.split < 2 || j.n % j.split || j.split > cap || j.n / j.split > cap)
+        // a power-of-two factor may use the compile-time kernels' larger cap
+        auto fits = [&](int64_t f) { return f <= (is_pow2(f) ? (int64_t)max_single_pass(prec) : (int64_t)cap); };
+        if (j.split < 2 || j.n % j.split || !fits(j.split) || !fits(j.n / j.split))
             return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step split must divide n into LDS-sized factors");
         n1 = j.split;
     }
@@ -619,14 +620,26 @@ static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& alg
     const size_t need = (size_t)(cc * j.n * es);
     if (need > p.tmp_bytes[0]) p.tmp_bytes[0] = need;
     int rc;
+    // A power-of-two factor held in LDS runs on the compile-time Stockham
+    // kernel of that length, the other factor on the runtime passes.
+    const KernelInfo* ka = (is_pow2(n1) && n1 >= 2) ? find_kernel(prec, (int)n1, VAR_4A, dir) : nullptr;
+    const KernelInfo* kb = (is_pow2(n2) && n2 >= 2) ? find_kernel(prec, (int)n2, VAR_4B, dir) : nullptr;
     GArgs ga{}, gb{};
-    int tpa, ldsa, tha, tpb, ldsb, thb;
-    if ((rc = gen_args(p, n1, false, ga, &tpa, &ldsa, &tha))) return rc;
-    if ((rc = gen_args(p, n2, false, gb, &tpb, &ldsb, &thb))) return rc;
+    int tpa = 1, ldsa = 0, tha = 1, tpb = 1, ldsb = 0, thb = 1;
+    if (ka) {
+        if ((rc = schedule_twiddles(p, ka, &ga.k.tw))) return rc;
+    } else if ((rc = gen_args(p, n1, false, ga, &tpa, &ldsa, &tha))) {
+        return rc;
+    }
+    if (kb) {
+        if ((rc = schedule_twiddles(p, kb, &gb.k.tw))) return rc;
+    } else if ((rc = gen_args(p, n2, false, gb, &tpb, &ldsb, &thb))) {
+        return rc;
+    }
     if ((rc = split_twiddles(p, prec, j.n, &ga.k.tw4_shift, &ga.k.tw4_lo, &ga.k.tw4_hi))) return rc;
-    const void* fa = gen_kernel(prec, VAR_4A, dir);
-    const void* fb = gen_kernel(prec, VAR_4B, dir);
-    if ((rc = prep_dynamic_lds(fa, ldsa)) || (rc = prep_dynamic_lds(fb, ldsb))) return rc;
+    const void* fa = ka ? ka->fn : gen_kernel(prec, VAR_4A, dir);
+    const void* fb = kb ? kb->fn : gen_kernel(prec, VAR_4B, dir);
+    if ((!ka && (rc = prep_dynamic_lds(fa, ldsa))) || (!kb && (rc = prep_dynamic_lds(fb, ldsb)))) return rc;
     ga.k.tw4_L = j.n;
     ga.k.count = n2;
     ga.k.in_cd = 1; ga.k.in_nd = n2; ga.k.out_cd = 1; ga.k.out_nd = n2;
@@ -643,13 +656,15 @@ static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& alg
             GArgs a2 = ga, b2 = gb;
             a2.k.in_bd = j.in_cd;
             b2.k.out_bd = j.out_cd;
-            Step sa = make_gen_step(fa, a2, (n2 + Ga - 1) / Ga, ldsa, tha, "gen4A");
+            Step sa = ka ? make_step(ka, a2.k, (n2 + ka->tpb - 1) / ka->tpb, 1, "4stepA")
+                         : make_gen_step(fa, a2, (n2 + Ga - 1) / Ga, ldsa, tha, "gen4A");
             sa.in_role = j.in_role;
             sa.in_off = j.in_off + (y * j.in_bd + c0 * j.in_cd) * es;
             sa.out_role = ROLE_TMP0;
             sa.out_off = 0;
             push_sliced(p.steps[di], sa, cnt, j.in_cd * es, j.n * es);
-            Step sb = make_gen_step(fb, b2, (n1 + Gb - 1) / Gb, ldsb, thb, "gen4B");
+            Step sb = kb ? make_step(kb, b2.k, (n1 + kb->tpb - 1) / kb->tpb, 1, "4stepB")
+                         : make_gen_step(fb, b2, (n1 + Gb - 1) / Gb, ldsb, thb, "gen4B");
             sb.in_role = ROLE_TMP0;
             sb.in_off = 0;
             sb.out_role = j.out_role;
@@ -695,48 +710,73 @@ static void host_fft_ld(std::vector<long double>& re, std::vector<long double>& 
     }
 }
 
-int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt, int64_t tr_m1) {
-    const bool refchirp = p.desc.flags & AFFT_FLAG_REF_CHIRP;
-    std::vector<double> ch(2 * n);
-    for (int64_t k = 0; k < n; ++k) {
-        if (refchirp) {
+// Forward spectrum of one sequence of m = 2^k complex128 points (interleaved
+// re, im in `v`, replaced by its DFT), on the plan's device with a complex128
+// plan of that length forced onto the power-of-two routes (never Bluestein).
+static int device_fft_c128(Plan& p, std::vector<double>& v, int64_t m) {
+    afft_plan_desc d{};
+    d.kind = AFFT_C2C;
+    d.precision = AFFT_COMPLEX128;
+    d.rank = 1;
+    d.dims[0] = m;
+    d.batch = 1;
+    d.strategy = m <= max_single_pass(1) ? AFFT_STRATEGY_STOCKHAM : AFFT_STRATEGY_SIXSTEP;
+    afft_plan sub = nullptr;
+    int rc = afft_plan_create_desc(static_cast<afft_context>(p.ctx), &d, &sub);
+    if (rc) return rc;
+    const size_t bytes = (size_t)m * 16;
+    void* buf = nullptr;
+    hipError_t e = hipMalloc(&buf, 2 * bytes);
+    if (e == hipSuccess) e = hipMemcpy(buf, v.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = exec_plan(*sub, AFFT_FORWARD, (char*)buf + bytes, buf, sub->stream);
+        if (rc == AFFT_OK) e = hipStreamSynchronize(sub->stream);
+        if (rc == AFFT_OK && e == hipSuccess) e = hipMemcpy(v.data(), (char*)buf + bytes, bytes, hipMemcpyDeviceToHost);
+    }
+    if (buf) (void)hipFree(buf);
+    afft_plan_destroy(sub);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_error(AFFT_ERR_DEVICE, std::string("Bluestein filter spectrum: ") + hipGetErrorString(e));
+    return AFFT_OK;
+}
+
+// The chirp exp(-iπ k²/n) = W_{2n}^{k² mod 2n}, k < n, in FP64.  Above 4096
+// points from two long-double tables of about sqrt(2n) roots each,
+// W_{2n}^r = W_{2n}^{(r >> s) 2^s} W_{2n}^{r & (2^s - 1)}, multiplied in long
+// double and rounded once (the product's error is far below a double ulp).
+// AFFT_FLAG_REF_CHIRP uses the reference's unreduced argument π k²/n.
+static void bluestein_chirp(int64_t n, bool refchirp, std::vector<double>& ch) {
+    ch.assign(2 * n, 0.0);
+    if (refchirp) {
+        for (int64_t k = 0; k < n; ++k) {
             const double a = -3.14159265358979323846 * (double)(k * k) * (1.0 / (double)n);
             ch[2 * k] = std::cos(a);
             ch[2 * k + 1] = std::sin(a);
-        } else {
-            const unsigned __int128 kk = (unsigned __int128)k * (unsigned __int128)k;
-            const int64_t r = (int64_t)(kk % (unsigned __int128)(2 * n));
-            root_of_unity(r, 2 * n, &ch[2 * k], &ch[2 * k + 1]);   // exp(-2πi r/(2N))
         }
+        return;
     }
-    std::vector<double> filt[2];
-    for (int d = 0; d < 2; ++d) {
-        std::vector<long double> re(m, 0.0L), im(m, 0.0L);
-        // forward: b[k] = conj(chirp[k]); inverse: b[k] = chirp[k]; mirrored
-        const long double sgn = d == 0 ? -1.0L : 1.0L;
-        re[0] = ch[0];
-        im[0] = sgn * ch[1];
-        for (int64_t k = 1; k < n; ++k) {
-            re[k] = re[m - k] = ch[2 * k];
-            im[k] = im[m - k] = sgn * ch[2 * k + 1];
-        }
-        host_fft_ld(re, im);
-        filt[d].resize(2 * m);
-        // natural order, or row-major [k1][k2] of k = k1 + tr_m1 * k2
-        const int64_t m2 = tr_m1 > 0 ? m / tr_m1 : 0;
-        for (int64_t k = 0; k < m; ++k) {
-            const int64_t at = tr_m1 > 0 ? (k % tr_m1) * m2 + k / tr_m1 : k;
-            filt[d][2 * at] = (double)(re[k] / (long double)m);
-            filt[d][2 * at + 1] = (double)(im[k] / (long double)m);
-        }
+    const int64_t L = 2 * n;
+    if (n <= 4096) {
+        for (int64_t k = 0, r = 0; k < n; r = (r + 2 * k + 1) % L, ++k)
+            root_of_unity(r, L, &ch[2 * k], &ch[2 * k + 1]);   // r = k^2 mod 2n
+        return;
     }
-    std::vector<double> post[2];
-    post[0] = ch;
-    post[1].resize(2 * n);
-    for (int64_t k = 0; k < n; ++k) {
-        post[1][2 * k] = ch[2 * k] * inv_scale;
-        post[1][2 * k + 1] = -ch[2 * k + 1] * inv_scale;
+    int sh = 0;
+    while ((int64_t(1) << (2 * sh)) < L) ++sh;
+    const int64_t B = int64_t(1) << sh, H = (L + B - 1) / B;
+    std::vector<long double> lr(B), li(B), hr(H), hi(H);
+    for (int64_t i = 0; i < B; ++i) root_of_unity_ld(i, L, &lr[i], &li[i]);
+    for (int64_t i = 0; i < H; ++i) root_of_unity_ld(i * B, L, &hr[i], &hi[i]);
+    for (int64_t k = 0, r = 0; k < n; r = (r + 2 * k + 1) % L, ++k) {
+        const int64_t a = r & (B - 1), b = r >> sh;
+        ch[2 * k] = (double)(lr[a] * hr[b] - li[a] * hi[b]);
+        ch[2 * k + 1] = (double)(lr[a] * hi[b] + li[a] * hr[b]);
     }
+}
+
+int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt, int64_t tr_m1) {
+    const bool refchirp = p.desc.flags & AFFT_FLAG_REF_CHIRP;
+    int rc;
     auto up = [&](const std::vector<double>& v, const void** dev) -> int {
         if (p.desc.precision == 0) {
             std::vector<float> f(v.begin(), v.end());
@@ -744,13 +784,69 @@ int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt,
         }
         return upload_table(p, v.data(), v.size() * 8, dev);
     };
-    int rc;
-    if ((rc = up(ch, &bt.chirp))) return rc;
-    for (int d = 0; d < 2; ++d) {
-        if ((rc = up(filt[d], &bt.filt[d]))) return rc;
-        if ((rc = up(post[d], &bt.post[d]))) return rc;
+    // chirp and filter spectra are shared by the plan's two directions
+    const auto key = std::make_tuple(n, m, tr_m1);
+    auto it = p.blue_cache.find(key);
+    if (it == p.blue_cache.end()) {
+        BlueCache bc;
+        bluestein_chirp(n, refchirp, bc.ch);
+        // Forward filter b[k] = conj(chirp[k]), mirrored (b[m-k] = b[k]), zero
+        // between.  Its spectrum B: a long-double host FFT up to 4096 points,
+        // above that one complex128 FFT on the device (each costs about
+        // 0.1-0.5 ms around 4096 points).  The inverse filter is
+        // conj(b), whose spectrum is conj(B[-k mod m]).
+        std::vector<double> spec(2 * m, 0.0);
+        spec[0] = bc.ch[0];
+        spec[1] = -bc.ch[1];
+        for (int64_t k = 1; k < n; ++k) {
+            spec[2 * k] = spec[2 * (m - k)] = bc.ch[2 * k];
+            spec[2 * k + 1] = spec[2 * (m - k) + 1] = -bc.ch[2 * k + 1];
+        }
+        if (m <= 4096) {
+            std::vector<long double> re(m), im(m);
+            for (int64_t k = 0; k < m; ++k) {
+                re[k] = spec[2 * k];
+                im[k] = spec[2 * k + 1];
+            }
+            host_fft_ld(re, im);
+            for (int64_t k = 0; k < m; ++k) {
+                spec[2 * k] = (double)re[k];
+                spec[2 * k + 1] = (double)im[k];
+            }
+        } else if ((rc = device_fft_c128(p, spec, m))) {
+            return rc;
+        }
+        std::vector<double> filt[2];
+        // natural order, or row-major [k1][k2] of k = k1 + tr_m1 * k2
+        const int64_t m2 = tr_m1 > 0 ? m / tr_m1 : 0;
+        const double inv_m = 1.0 / (double)m;   // m is a power of two: exact
+        for (int d = 0; d < 2; ++d) filt[d].resize(2 * m);
+        for (int64_t k = 0; k < m; ++k) {
+            const int64_t at = tr_m1 > 0 ? (k % tr_m1) * m2 + k / tr_m1 : k;
+            const int64_t mk_ = (m - k) & (m - 1);
+            filt[0][2 * at] = spec[2 * k] * inv_m;
+            filt[0][2 * at + 1] = spec[2 * k + 1] * inv_m;
+            filt[1][2 * at] = spec[2 * mk_] * inv_m;
+            filt[1][2 * at + 1] = -spec[2 * mk_ + 1] * inv_m;
+        }
+        if ((rc = up(bc.ch, &bc.chirp))) return rc;
+        for (int d = 0; d < 2; ++d)
+            if ((rc = up(filt[d], &bc.filt[d]))) return rc;
+        it = p.blue_cache.emplace(key, std::move(bc)).first;
     }
-    return AFFT_OK;
+    const std::vector<double>& ch = it->second.ch;
+    bt.chirp = it->second.chirp;
+    bt.filt[0] = it->second.filt[0];
+    bt.filt[1] = it->second.filt[1];
+    // post-multiply tables: the chirp itself (forward), conj(chirp) times the
+    // inverse scale (inverse)
+    bt.post[0] = bt.chirp;
+    std::vector<double> post(2 * n);
+    for (int64_t k = 0; k < n; ++k) {
+        post[2 * k] = ch[2 * k] * inv_scale;
+        post[2 * k + 1] = -ch[2 * k + 1] * inv_scale;
+    }
+    return up(post, &bt.post[1]);
 }
 
 static int emit_bluestein(Plan& p, int di, const AxisJob& j, std::string& algo) {
@@ -1160,12 +1256,14 @@ static int emit_real_rows(Plan& p, int di, int64_t n, int64_t rows, int64_t slic
 // MFMA plans keep the MFMA routing: wisdom and the cost model rank the
 // plain kernels, and their picks (mixed radix, Bluestein, other four-step
 // splits) have no MFMA form.
+uint64_t wisdom_features(const Context& c, const afft_plan_desc& d);   // tune.cpp
+
 void choose_strategy(Plan& p, AxisJob& j) {
     if (j.in_nd != 1 || j.out_nd != 1) return;
     if (p.desc.flags & AFFT_FLAG_MFMA) return;
     Context* c = p.ctx;
     std::lock_guard<std::mutex> lock(c->mu);
-    auto it = c->wisdom.find(std::make_tuple(j.n, p.desc.precision, c->features()));
+    auto it = c->wisdom.find(std::make_tuple(j.n, p.desc.precision, wisdom_features(*c, p.desc)));
     Candidate pick{0, 0};
     if (it != c->wisdom.end()) {
         if (parse_strategy(it->second.algorithm, &pick)) {

@@ -30,6 +30,8 @@ using namespace afft;
 
 namespace afft {
 const void* rag_p2_kernel(int prec, int kind, int L, int dir);   // ragged_p2.hip
+const void* rag_whole_kernel(int prec, int kind, int L, int dir, int* threads);   // ragged_whole.hip
+int rag_whole_p2_min();
 }
 
 namespace afft {
@@ -41,19 +43,22 @@ __global__ void k_rag_reset(int* counter) {
 }  // namespace afft
 
 // The launches form independent chains (the tasks of a job depend only on
-// earlier launches of the same chain): generic smooth splits, power-of-two
-// splits, Bluestein splits, whole jobs.
-constexpr int RAG_BRANCHES = 4;
+// earlier launches of the same chain): runtime-pass lengths, power-of-two
+// lengths, Bluestein jobs.  Each chain runs its split jobs' launches, then
+// its whole jobs' launches (whole jobs depend on nothing, so they fill the
+// chain's tail).  Measured on the 4096-job mixed set: 0.561 ms with the
+// whole-job launches on these three chains, against 0.604 ms with them on a
+// fourth chain of their own (best shares of each layout).
+constexpr int RAG_BRANCHES = 3;
 static int rag_branch(int kind) {
     switch (kind) {
     case RL_COLS:
-    case RL_ROWS: return 0;
+    case RL_ROWS:
+    case RL_WHOLE: return 0;
     case RL_COLS_P2:
-    case RL_ROWS_P2: return 1;
-    case RL_COLS_BLUE:
-    case RL_ROWS_BLUE:
-    case RL_COLS2: return 2;
-    default: return 3;
+    case RL_ROWS_P2:
+    case RL_WHOLE_P2: return 1;
+    default: return 2;   // RL_COLS_BLUE, RL_ROWS_BLUE, RL_COLS2, RL_WHOLE_BLUE
     }
 }
 
@@ -63,6 +68,7 @@ static int rag_branch(int kind) {
 struct RagLaunch {
     int kind = 0, L = 0;
     int ntask = 0, task_off = 0;
+    int threads = RAG_THREADS;
     int grid[2] = {0, 0};   // [inverse]
 };
 
@@ -138,7 +144,7 @@ static int ragged_order_end(afft_ragged_s& r, hipStream_t s, bool track) {
     return AFFT_OK;
 }
 
-// Radix schedule and per-pass twiddles of length m (m <= RAG_WHOLE), shared
+// Radix schedule and per-pass twiddles of length m (m <= rag_whole_cap), shared
 // between jobs through `cache`.  The table is r-major (entry (r-1)*Ns + k,
 // the layout of the compile-time cores) for the power-of-two split tasks and
 // k-major (entry k*(R-1) + r-1, one base address per butterfly) for the
@@ -224,15 +230,23 @@ static const void* rag_kernel_t(int kind, int dir) {
     switch (kind) {
     case RL_COLS: return dir > 0 ? (const void*)&k_rag_cols<T, 1> : (const void*)&k_rag_cols<T, -1>;
     case RL_ROWS: return dir > 0 ? (const void*)&k_rag_rows<T, 1> : (const void*)&k_rag_rows<T, -1>;
-    case RL_WHOLE: return dir > 0 ? (const void*)&k_rag_whole<T, 1> : (const void*)&k_rag_whole<T, -1>;
-    case RL_WHOLE_BLUE: return (const void*)&k_rag_whole_blue<T>;
     default: return nullptr;
     }
 }
 
+static bool whole_kind(int kind) { return kind == RL_WHOLE_P2 || kind == RL_WHOLE || kind == RL_WHOLE_BLUE; }
+
 static const void* rag_kernel(int prec, const RagLaunch& l, int dir) {
+    int th = 0;
+    if (whole_kind(l.kind)) return rag_whole_kernel(prec, l.kind, l.L, dir, &th);
     if (l.L) return rag_p2_kernel(prec, l.kind, l.L, dir);
     return prec ? rag_kernel_t<double>(l.kind, dir) : rag_kernel_t<float>(l.kind, dir);
+}
+
+static int rag_threads(int prec, const RagLaunch& l) {
+    int th = RAG_THREADS;
+    if (whole_kind(l.kind)) (void)rag_whole_kernel(prec, l.kind, l.L, -1, &th);
+    return th;
 }
 
 // A split runs on the power-of-two cores when both lengths are powers of two
@@ -250,8 +264,11 @@ static int ragged_build(afft_ragged_s& r) {
     const int es = r.prec ? 16 : 8;
     std::vector<RJob> jobs(r.sizes.size());
     std::map<SchedKey, RSched> scache;
-    std::map<int, std::vector<int>> groups;   // whole power-of-two jobs by length
+    std::map<int, std::vector<int>> groups;       // whole power-of-two jobs by length
     std::map<int, std::vector<int>> gen_groups;   // other whole plain jobs by length
+    std::map<int, std::vector<int>> blue_groups;  // whole Bluestein jobs by padded length
+    const int wpts = r.prec ? RagW<double>::PTS : RagW<float>::PTS;   // points per whole-job task
+    const int wth = r.prec ? RagW<double>::THREADS : RagW<float>::THREADS;
     std::map<std::pair<int64_t, int64_t>, BlueTables> bcache;
     struct TwSplit { int shift; const void* lo; const void* hi; };
     std::map<int64_t, TwSplit> tcache;
@@ -273,19 +290,25 @@ static int ragged_build(afft_ragged_s& r) {
         off += n;
         j.n = (int)n;
         int64_t m = n, m1 = 0, m2 = 0;
+        int64_t cap = rag_whole_cap(r.prec);
         bool blue = !(n == 1 || !smooth_schedule(n).empty());
-        if (!blue && m > RAG_WHOLE && !split_len(m, &m1, &m2)) blue = true;   // no usable split
+        if (!blue && m > cap && !split_len(m, &m1, &m2)) blue = true;   // no usable split
         if (blue) {
+            // Bluestein paddings above 8192 points split: a whole one holds
+            // two transforms' live values and spills at 1024 threads
+            // (measured 0.555 against 0.560 ms on the 4096-job set)
+            cap = std::min<int64_t>(cap, 8192);
             m = next_pow2(2 * n - 1);
-            if (m > RAG_WHOLE) split_len(m, &m1, &m2);
+            if (m > cap) split_len(m, &m1, &m2);
         }
         j.m = (int)m;
         j.blue = blue;
-        const bool whole = m <= RAG_WHOLE;
+        const bool whole = m <= cap;
         int rc;
-        // whole plain power-of-two jobs of 16 points or more run grouped on
-        // the compile-time cores (r-major table), the rest on runtime passes
-        const bool whole_p2 = whole && !blue && m >= 16 && (m & (m - 1)) == 0;
+        // whole power-of-two jobs and Bluestein paddings of rag_whole_p2_min()
+        // points or more run on the compile-time cores (r-major table), the
+        // rest on the runtime passes
+        const bool whole_p2 = whole && m >= rag_whole_p2_min() && (m & (m - 1)) == 0;
         if (whole) {
             if ((rc = schedule_into(r.tables, r.prec, m, whole_p2, j.s[0], scache))) return rc;
         } else {
@@ -326,18 +349,19 @@ static int ragged_build(afft_ragged_s& r) {
         }
         const int job = (int)i;
         if (whole_p2) {
-            groups[(int)m].push_back(job);
+            (blue ? blue_groups : groups)[(int)m].push_back(job);
             continue;
         }
-        if (whole && !blue) {
-            // RAG_PTS points per task at most, TPT = RAG_THREADS/G threads a job
-            j.g_col = pow2_floor(std::min<int64_t>(RAG_THREADS, RAG_PTS / m));
-            gen_groups[(int)m].push_back(job);
+        if (whole && blue) {
+            // one job per block on the runtime passes
+            ph[{RL_WHOLE_BLUE, 0}].push_back({{job, RT_WHOLE, 0, 1}, 2.0 * m * std::log2((double)m)});
             continue;
         }
         if (whole) {
-            const double c = (blue ? 2.0 : 1.0) * m * std::log2(std::max<double>(2.0, (double)m));
-            ph[{blue ? RL_WHOLE_BLUE : RL_WHOLE, 0}].push_back({{job, RT_WHOLE, 0, 1}, c});
+            // wpts points per task at most, TPT = wth/G threads a job (at
+            // most 16 points per thread)
+            j.g_col = pow2_floor(std::min<int64_t>(wth, wpts / m));
+            gen_groups[(int)m].push_back(job);
             continue;
         }
         // Power-of-two splits (every Bluestein job) run on the compile-time
@@ -362,25 +386,21 @@ static int ragged_build(afft_ragged_s& r) {
         }
     }
     r.total = off;
-    // grouped whole jobs: RAG_PTS/L jobs of length L per task, ids in joblist
+    // grouped whole jobs, ids in joblist: max(1, RAG_PTS/L) jobs of length L
+    // per task on the compile-time cores (one launch per length), g_col on
+    // the runtime passes (one launch for every length)
     std::vector<int> joblist;
-    for (auto& g : groups) {
-        const int L = g.first, per = RAG_PTS / L;
-        for (size_t q = 0; q < g.second.size(); q += per) {
-            const int cnt = (int)std::min<size_t>(per, g.second.size() - q);
-            const int first = (int)joblist.size();
-            joblist.insert(joblist.end(), g.second.begin() + q, g.second.begin() + q + cnt);
-            ph[{RL_WHOLE_P2, L}].push_back({{g.second[q], RT_WHOLE, first, cnt}, (double)cnt});
-        }
-    }
-    for (auto& g : gen_groups) {
-        const int L = g.first, per = jobs[g.second[0]].g_col;
-        const double c = L * std::log2(std::max(2.0, (double)L));
-        for (size_t q = 0; q < g.second.size(); q += per) {
-            const int cnt = (int)std::min<size_t>(per, g.second.size() - q);
-            const int first = (int)joblist.size();
-            joblist.insert(joblist.end(), g.second.begin() + q, g.second.begin() + q + cnt);
-            ph[{RL_WHOLE, 0}].push_back({{g.second[q], RT_WHOLE, first, cnt}, c * cnt});
+    for (int kind : {RL_WHOLE_P2, RL_WHOLE_BLUE, RL_WHOLE}) {
+        for (auto& g : kind == RL_WHOLE_P2 ? groups : kind == RL_WHOLE_BLUE ? blue_groups : gen_groups) {
+            const int L = g.first;
+            const int per = kind == RL_WHOLE ? jobs[g.second[0]].g_col : std::max(1, RAG_PTS / L);
+            const double c = (kind == RL_WHOLE_BLUE ? 2.0 : 1.0) * L * std::log2(std::max(2.0, (double)L));
+            for (size_t q = 0; q < g.second.size(); q += per) {
+                const int cnt = (int)std::min<size_t>(per, g.second.size() - q);
+                const int first = (int)joblist.size();
+                joblist.insert(joblist.end(), g.second.begin() + q, g.second.begin() + q + cnt);
+                ph[{kind, kind == RL_WHOLE ? 0 : L}].push_back({{g.second[q], RT_WHOLE, first, cnt}, c * cnt});
+            }
         }
     }
     // longest-processing-time order within each phase
@@ -410,12 +430,12 @@ static int ragged_build(afft_ragged_s& r) {
     AFFT_HIP(hipMalloc(&r.counter, RAG_COUNTERS * sizeof(int)));
     // Each persistent launch gets a share of the device's resident blocks,
     // so that the concurrent branches run side by side instead of the first
-    // one filling every CU until its queue drains.  Weights 4:4:6:3 (split
-    // smooth, split power-of-two, Bluestein chain, whole jobs) over the
-    // branches present, 1.7 device-fulls in all, at most one device-full per
-    // branch.  On the 4096-job mixed set this takes a graph execution from
-    // 0.600 to 0.572 ms; a set with one branch keeps the whole device.
-    static constexpr int weight[RAG_BRANCHES] = {4, 4, 6, 3};
+    // one filling every CU until its queue drains.  Weights 4:5:7 (runtime
+    // passes, power-of-two cores, Bluestein chain) over the branches
+    // present, 1.7 device-fulls in all, at most one device-full per branch;
+    // a set with one branch keeps the whole device.  Shares swept on the
+    // 4096-job mixed set: 0.561-0.61 ms over the settings tried.
+    static constexpr int weight[RAG_BRANCHES] = {4, 5, 7};
     bool present[RAG_BRANCHES] = {};
     for (const RagLaunch& l : r.launches) present[rag_branch(l.kind)] = true;
     int wsum = 0;
@@ -424,8 +444,9 @@ static int ragged_build(afft_ragged_s& r) {
         for (RagLaunch& l : r.launches) {
             const void* fn = rag_kernel(r.prec, l, inv ? 1 : -1);
             if (!fn) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no ragged kernel for a task type");
+            l.threads = rag_threads(r.prec, l);
             int per_cu = 0;
-            AFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, RAG_THREADS, 0));
+            AFFT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, l.threads, 0));
             const int full = std::max(1, per_cu) * std::max(1, r.ctx->props.multiProcessorCount);
             const double share = std::min(1.0, 1.7 * weight[rag_branch(l.kind)] / std::max(1, wsum));
             l.grid[inv] = std::max(1, (int)(full * share));
@@ -447,7 +468,7 @@ static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, 
         if (used[b]) AFFT_HIP(hipStreamWaitEvent(r.branch[b - 1], r.fork, 0));
     // Enqueue the longest chain first (Bluestein: three dependent passes),
     // then the others; each branch keeps its own launch order.
-    static constexpr int order[RAG_BRANCHES] = {2, 1, 0, 3};
+    static constexpr int order[RAG_BRANCHES] = {2, 1, 0};
     for (int ob = 0; ob < RAG_BRANCHES; ++ob)
     for (size_t p = 0; p < r.launches.size(); ++p) {
         const RagLaunch& l = r.launches[p];
@@ -466,7 +487,7 @@ static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, 
         a.scratch = r.scratch;
         void* args[] = {&a};
         const int grid = std::min(l.grid[dir > 0 ? 1 : 0], l.ntask);
-        AFFT_HIP(hipLaunchKernel(rag_kernel(r.prec, l, dir), dim3(grid), dim3(RAG_THREADS), args, 0, bs));
+        AFFT_HIP(hipLaunchKernel(rag_kernel(r.prec, l, dir), dim3(grid), dim3(l.threads), args, 0, bs));
     }
     for (int b = 1; b < RAG_BRANCHES; ++b)
         if (used[b]) {

@@ -23,8 +23,6 @@ static const void* p2_kernel_l(int kind, int dir) {
         default: break;
         }
     }
-    if (kind == RL_WHOLE_P2)
-        return dir > 0 ? (const void*)&k_rag_whole_p2<T, L, 1> : (const void*)&k_rag_whole_p2<T, L, -1>;
     return nullptr;
 }
 
@@ -44,8 +42,8 @@ static const void* p2_kernel_t(int kind, int L, int dir) {
     }
 }
 
-// The kernel of a power-of-two launch (RL_COLS_P2, RL_COLS_BLUE, RL_ROWS_P2,
-// RL_ROWS_BLUE, RL_COLS2, RL_WHOLE_P2) for (sub-)transforms of length L, or
+// The kernel of a power-of-two split launch (RL_COLS_P2, RL_COLS_BLUE,
+// RL_ROWS_P2, RL_ROWS_BLUE, RL_COLS2) for sub-transforms of length L, or
 // nullptr.
 const void* rag_p2_kernel(int prec, int kind, int L, int dir) {
     return prec ? p2_kernel_t<double>(kind, L, dir) : p2_kernel_t<float>(kind, L, dir);

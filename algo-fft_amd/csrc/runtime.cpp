@@ -56,27 +56,27 @@ int max_single_pass(int prec) {
 // ---- roots of unity ---------------------------------------------------------
 // cos/sin of θ = 2π m/L with m reduced in integer arithmetic and θ folded
 // into [0, π/4] before an extended-precision cosl/sinl.
-static void cossin2pi(int64_t m, int64_t L, double* c, double* s) {
+static void cossin2pi(int64_t m, int64_t L, long double* c, long double* s) {
     m %= L;
     if (m < 0) m += L;
     const int64_t m8 = 8 * m;  // L < 2^59
     if (m8 <= L) {
         const long double two_pi = 6.283185307179586476925286766559005768L;
         long double a = two_pi * (long double)m / (long double)L;
-        *c = (double)cosl(a);
-        *s = (double)sinl(a);
+        *c = cosl(a);
+        *s = sinl(a);
     } else if (m8 <= 2 * L) {  // θ = π/2 - φ, φ = 2π (L-4m)/(4L)
-        double c2, s2;
+        long double c2, s2;
         cossin2pi(L - 4 * m, 4 * L, &c2, &s2);
         *c = s2;
         *s = c2;
     } else if (m8 <= 4 * L) {  // θ = π - φ, φ = 2π (L-2m)/(2L)
-        double c2, s2;
+        long double c2, s2;
         cossin2pi(L - 2 * m, 2 * L, &c2, &s2);
         *c = -c2;
         *s = s2;
     } else {                   // θ = 2π - φ, φ = 2π (L-m)/L
-        double c2, s2;
+        long double c2, s2;
         cossin2pi(L - m, L, &c2, &s2);
         *c = c2;
         *s = -s2;
@@ -84,7 +84,14 @@ static void cossin2pi(int64_t m, int64_t L, double* c, double* s) {
 }
 
 void root_of_unity(int64_t m, int64_t L, double* re, double* im) {
-    double c, s;
+    long double c, s;
+    cossin2pi(m, L, &c, &s);
+    *re = (double)c;
+    *im = (double)-s;
+}
+
+void root_of_unity_ld(int64_t m, int64_t L, long double* re, long double* im) {
+    long double c, s;
     cossin2pi(m, L, &c, &s);
     *re = c;
     *im = -s;

@@ -48,11 +48,35 @@ std::vector<Candidate> plan_candidates(const afft_plan_desc& d) {
             return std::fabs(std::log((double)a * a / n)) < std::fabs(std::log((double)b * b / n));
         });
         for (size_t i = 0; i < divs.size() && i < 3; ++i) out.push_back({AFFT_STRATEGY_SIXSTEP, divs[i]});
+        // the splits whose column or row factor is the power-of-two part of
+        // n: that pass runs on the compile-time Stockham kernel
+        const int64_t p2 = n & -n;
+        for (int64_t s : {p2, n / p2}) {
+            const int64_t o = n / s;
+            if (p2 < 16 || s > (s == p2 ? maxp : gcap) || o > (o == p2 ? maxp : gcap)) continue;
+            bool seen = false;
+            for (const Candidate& c : out) seen |= c.strategy == AFFT_STRATEGY_SIXSTEP && c.split == s;
+            if (!seen) out.push_back({AFFT_STRATEGY_SIXSTEP, s});
+        }
     }
     // Bluestein is the only route for other sizes; for smooth sizes it is a
     // candidate while its padded length stays modest
     if (!smooth || (!is_pow2(n) && 2 * n - 1 <= (1 << 20))) out.push_back({AFFT_STRATEGY_BLUESTEIN, 0});
     return out;
+}
+
+// Wisdom feature word of a plan: the device's architecture number in the low
+// 16 bits (0x950) and the shape class above it, so that a winner timed at one
+// batch size and layout is applied only to plans of the same class: batch
+// bucket 0 (1), 1 (2-15), 2 (16-255), 3 (256-4095), 4 (4096 and more), plus 8
+// for strided layouts.  The reference's field holds CPU feature bits
+// (planner/wisdom.go:17-30); a file written without the class (features
+// 0x950) applies to single unit-stride transforms.
+uint64_t wisdom_features(const Context& c, const afft_plan_desc& d) {
+    const int64_t b = d.batch > 0 ? d.batch : 1;
+    const int bucket = b <= 1 ? 0 : b < 16 ? 1 : b < 256 ? 2 : b < 4096 ? 3 : 4;
+    const bool strided = d.istride > 1 || d.ostride > 1;
+    return c.features() | ((uint64_t)(bucket + (strided ? 8 : 0)) << 16);
 }
 
 std::string strategy_name(const Candidate& c) {
@@ -251,13 +275,18 @@ int afft_tune_plan(afft_context ctx, const afft_plan_desc* d, int warmup, int it
         WisdomEntry w;
         w.size = d->dims[0];
         w.precision = d->precision;
-        w.features = ctx->features();
+        w.features = wisdom_features(*ctx, *d);
         w.algorithm = strategy_name(win);
         w.timestamp = (int64_t)std::time(nullptr);
         ctx->wisdom[std::make_tuple(w.size, w.precision, w.features)] = w;
     }
     clear_error();
     return AFFT_OK;
+}
+
+uint64_t afft_wisdom_features(afft_context ctx, const afft_plan_desc* d) {
+    if (!ctx || !d) return 0;
+    return wisdom_features(*ctx, *d);
 }
 
 // Wisdom.Export: sorted lines "size:precision:features:algorithm:timestamp".

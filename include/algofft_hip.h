@@ -280,6 +280,12 @@ int afft_tune_time(afft_context ctx, const afft_plan_desc* desc, int warmup, int
  * wisdom in the context. */
 int afft_tune_plan(afft_context ctx, const afft_plan_desc* desc, int warmup, int iters, int* best_strategy,
                    int64_t* best_split, double* best_ms, double* times, int cap);
+/* Features word of the wisdom key for a plan description: the device's
+ * architecture number (0x950) in bits 0-15 and the shape class in bits 16-19
+ * (batch bucket 0: 1, 1: 2-15, 2: 16-255, 3: 256-4095, 4: 4096 and more;
+ * +8 for strided layouts), so that a winner is applied only to plans of the
+ * class it was timed for (the field of planner/wisdom.go:17-30). */
+uint64_t afft_wisdom_features(afft_context ctx, const afft_plan_desc* desc);
 int afft_wisdom_export(afft_context ctx, char* buf, int64_t cap, int64_t* needed);  /* Wisdom.Export */
 int afft_wisdom_import(afft_context ctx, const char* text);                          /* Wisdom.Import */
 int afft_wisdom_clear(afft_context ctx);

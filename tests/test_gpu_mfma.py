@@ -21,10 +21,11 @@ def _run(ctx, x, flags, inverse=False):
         return y, p.describe()
 
 
-@pytest.mark.parametrize("n", [256, 1024, 4096, 65536, 1 << 18, 1 << 20])
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048, 4096, 8192, 16384, 65536, 1 << 17, 1 << 18, 1 << 20,
+                               1 << 21, 1 << 22, 1 << 24])
 @pytest.mark.parametrize("inverse", [False, True])
 def test_mfma_matches_oracle(ctx, n, inverse):
-    batch = max(2, min(64, (1 << 21) // n))
+    batch = max(1, min(64, (1 << 21) // n))
     x = synth.uniform_complex(n, (batch, n), np.complex64)
     y, algo = _run(ctx, x, A.FLAG_MFMA, inverse)
     assert "mfma" in algo, algo
@@ -35,7 +36,7 @@ def test_mfma_matches_oracle(ctx, n, inverse):
     assert err <= c64_tol(n)
 
 
-@pytest.mark.parametrize("n", [256, 1024, 4096, 65536, 1 << 20])
+@pytest.mark.parametrize("n", [256, 1024, 2048, 4096, 16384, 65536, 1 << 20])
 def test_mfma_on_high_dynamic_range_tones(ctx, n):
     rows = 32 if n <= 4096 else 4
     x, bins, amps = synth.sinusoids(77, n, rows)
@@ -59,8 +60,26 @@ def test_mfma_strided_four_step_slices(ctx):
     assert rel_l2(y[:, :n], ref) <= c64_tol(n)
 
 
+def test_mfma_per_transform_scale(ctx):
+    """Each transform is scaled by its own power of two: rows of very
+    different magnitude in one block (1e-30 .. 1e30, and an all-zero row)
+    each keep the complex64 budget relative to themselves."""
+    n, batch = 256, 16
+    x = synth.uniform_complex(41, (batch, n), np.complex64)
+    mags = np.array([1e-30, 1e30, 0.0, 1.0, 3e-5, 7e4, 1e-3, 1e3] * 2, np.float64)
+    x = (x * mags[:, None]).astype(np.complex64)
+    y, algo = _run(ctx, x, A.FLAG_MFMA)
+    assert "mfma" in algo
+    ref = O.fft_batch(x.astype(np.complex128))
+    for b in range(batch):
+        if mags[b] == 0:
+            assert not np.any(y[b])
+        else:
+            assert rel_l2(y[b], ref[b]) <= c64_tol(n), (b, mags[b])
+
+
 def test_mfma_unsupported_shapes_are_reported(ctx):
-    for n in (2048, 8192, 1000, 1 << 22):
+    for n in (128, 1000, 1 << 25):
         with pytest.raises(A.ErrNotImplemented):
             A.Plan.from_desc(ctx, (n,), flags=A.FLAG_MFMA)
     with pytest.raises(A.ErrNotImplemented):
@@ -74,7 +93,8 @@ def test_mfma_plans_ignore_wisdom_for_the_plain_kernels():
     route and its result stays within budget; a plain plan follows the
     wisdom."""
     with A.Context(0) as c2:
-        text = f"{1 << 20}:0:{0x950}:hip_sixstep_512:1\n4096:0:{0x950}:hip_mixed:1\n"
+        # batch 2 is bucket 1 and batch 16 bucket 2 of the wisdom shape class
+        text = f"{1 << 20}:0:{0x950 | (1 << 16)}:hip_sixstep_512:1\n4096:0:{0x950 | (2 << 16)}:hip_mixed:1\n"
         A.check(A.lib().afft_wisdom_import(c2.h, text.encode()))
         for n, plain_algo in ((1 << 20, "fourstep512x2048"), (4096, "mixed4096r")):
             batch = 2 if n > 4096 else 16

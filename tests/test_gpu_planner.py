@@ -90,9 +90,13 @@ def test_wisdom_tune_export_import_and_use():
     text = _export(ctx)
     lines = text.strip().split("\n")
     assert len(lines) == 2
+    # features: the device word and the shape class (batch 4: bucket 1, batch 32: bucket 2)
+    want_feat = {1 << 18: 0x950 | (1 << 16), 3000: 0x950 | (2 << 16)}
+    for sh in shapes:
+        assert A.lib().afft_wisdom_features(ctx.h, ctypes.byref(planner.desc_for(sh))) == want_feat[sh["n"]]
     for ln in lines:
         size, prec, feat, algo, ts = ln.split(":")
-        assert int(size) in (1 << 18, 3000) and prec == "0" and int(feat) == 0x950 and int(ts) > 0
+        assert int(size) in (1 << 18, 3000) and prec == "0" and int(feat) == want_feat[int(size)] and int(ts) > 0
         assert algo.startswith("hip_")
     assert int(lines[0].split(":")[0]) < int(lines[1].split(":")[0])   # sorted by size
     ctx.close()
@@ -103,13 +107,16 @@ def test_wisdom_tune_export_import_and_use():
     with pytest.raises(A.ErrInvalidArgument):
         A.check(A.lib().afft_wisdom_import(ctx2.h, b"12:0:2384:hip_bogus:5\n"))
     forced = "hip_sixstep_256"
-    A.check(A.lib().afft_wisdom_import(ctx2.h, f"{1 << 18}:0:{0x950}:{forced}:1\n".encode()))
+    A.check(A.lib().afft_wisdom_import(ctx2.h, f"{1 << 18}:0:{0x950 | (1 << 16)}:{forced}:1\n".encode()))
     x = synth.uniform_complex(1, (4, 1 << 18), np.complex64)
     with A.Plan.from_desc(ctx2, (1 << 18,), batch=4) as p:
         assert p.describe() == "hip_fourstep256x1024", p.describe()
         y = np.empty_like(x)
         p.forward(y, x)
     assert rel_l2(y, O.fft_batch(x.astype(np.complex128))) <= c64_tol(1 << 18)
+    # the entry was timed for batches 2-15: a batch of 64 keeps the built-in route
+    with A.Plan.from_desc(ctx2, (1 << 18,), batch=64) as p:
+        assert p.describe() != "hip_fourstep256x1024", p.describe()
     ctx2.close()
 
 
@@ -166,3 +173,33 @@ def test_model_attached_to_context_plans_correctly():
     A.check(A.lib().afft_context_set_cost_model(ctx.h, None))
     m.close()
     ctx.close()
+
+
+@pytest.mark.skipif(not os.path.exists(RECORDS), reason="tuner records not collected")
+def test_cost_model_regret_on_freshly_timed_shapes(ctx):
+    """Regret on shapes the records never saw, timed now by the autotuner
+    (afft_tune_plan, every candidate): 300 seeded shapes from the planner
+    workload's distribution with a seed the records do not use.  The network
+    is trained on all stored records with three seeds; each must stay within
+    10 % mean regret, and their mean must not lose to the built-in routing
+    (the method without the network) on the same fresh timings."""
+    g_rec = planner.group(planner.load_records(RECORDS))
+    X, y = planner.dataset(g_rec, sorted(g_rec))
+    shapes = planner.tuner_shapes(9001, 300)
+    rows = planner.collect(ctx, shapes, 1, 3)
+    g = planner.group(rows)
+    keys = [k for k in sorted(g) if len(g[k][1]) > 1]
+    assert len(keys) >= 200
+    r_heur, r_heur_max = planner.regret(g, keys, planner.heuristic_pick)
+    res = []
+    for seed in (11, 12, 13):
+        m = planner.CostModel(ctx, seed=seed)
+        m.fit(X, y, epochs=200, batch=256, lr=2e-3, seed=seed)
+        res.append(planner.regret(g, keys, planner.model_pick(m)))
+        m.close()
+    mean_model = float(np.mean([r[0] for r in res]))
+    print(f"fresh shapes with a choice: {len(keys)}; built-in routing regret mean {r_heur:.4f} max {r_heur_max:.3f}; "
+          f"network regret per seed " + ", ".join(f"{r[0]:.4f} (max {r[1]:.3f})" for r in res) +
+          f"; mean {mean_model:.4f}")
+    assert all(r[0] <= 0.10 for r in res)
+    assert mean_model <= r_heur

@@ -148,3 +148,31 @@ def test_ragged_executions_on_several_streams(ctx):
         torch.cuda.synchronize()
         for y, r in zip(outs, refs):
             assert torch.equal(y, r)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_ragged_whole_jobs_up_to_the_lds_cap(ctx, dtype):
+    """Jobs of up to 16384 points (complex64; 8192 complex128) run whole, in
+    one task: power-of-two ones on blocks of L/16 threads, smooth ones on the
+    runtime passes with one job per block, and Bluestein jobs grouped by
+    padded length (jobs of one group differ in n, chirp and filter), on both
+    sides of the cap and beside split jobs."""
+    cap = 16384 if dtype == np.complex64 else 8192
+    primes_small = [11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97]
+    sizes = primes_small * 3 + [2053, 2063, 4099, 4111, 8191, 8209] + \
+        [8192, 16384, 4375, 6000, 10000, 15625, 16807, 4097 * 3, 32768, 20000] * 2 + [4096, 2048, 1000]
+    rng = np.random.default_rng(5)
+    sizes = [int(v) for v in rng.permutation(sizes)]
+    prec = A.COMPLEX64 if dtype == np.complex64 else A.COMPLEX128
+    with A.Ragged(ctx, sizes, prec) as rg:
+        offs = rg.offsets()
+        x = synth.uniform_complex(29, (len(rg),), dtype)
+        xt = torch.from_numpy(x).to("cuda:0")
+        s = torch.cuda.Stream()
+        for direction, inverse in ((A.FORWARD, False), (A.INVERSE, True)):
+            yt = torch.zeros_like(xt)
+            with torch.cuda.stream(s):
+                rg.exec_graph(direction, yt.data_ptr(), xt.data_ptr(), s.cuda_stream)
+            s.synchronize()
+            print("cap", cap, "inverse" if inverse else "forward", "worst err/tol",
+                  _check(sizes, offs, x, yt.cpu().numpy(), dtype, inverse=inverse))
