@@ -127,14 +127,17 @@ def heuristic_choice(shape, cands):
     elif smooth and n <= gcap:
         want = (STRATEGY_MIXED, 0)
     elif smooth:
-        want = None  # most balanced divisor
-        best = None
-        q = 1
-        while q * q <= n:
-            if n % q == 0 and q <= gcap and n // q <= gcap:
-                best = q
-            q += 1
-        want = (STRATEGY_SIXSTEP, best)
+        p2 = n & -n
+        if 16 <= p2 <= 4096 and n // p2 <= 2048:
+            want = (STRATEGY_SIXSTEP, p2)   # compile-time column pass
+        else:
+            best = None  # most balanced divisor
+            q = 1
+            while q * q <= n:
+                if n % q == 0 and q <= gcap and n // q <= gcap:
+                    best = q
+                q += 1
+            want = (STRATEGY_SIXSTEP, best)
     else:
         want = (STRATEGY_BLUESTEIN, 0)
     return want if want in cands else cands[0]

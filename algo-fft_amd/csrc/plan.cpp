@@ -606,6 +606,16 @@ static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& alg
     int64_t n1 = 0;
     for (int64_t d = 1; d * d <= j.n; ++d)
         if (j.n % d == 0 && d <= cap && j.n / d <= cap) n1 = d;
+    // The power-of-two part of n as the column factor, when it is 16 .. 4096
+    // points and the rest at most 2048: the column pass then runs on the
+    // compile-time kernel.  Over the 675 smooth four-step shapes of the
+    // tuning records this rule has 8.9 % mean regret against 12.3 % for the
+    // balanced split alone; N=12000 as 32 x 375 takes 0.62 ms against
+    // 0.76 ms as 100 x 120 on 512 MiB.  With a longer rest (100000 =
+    // 32 x 3125: 2.8 ms against 1.05 ms) or a long power-of-two column
+    // (344064 = 16384 x 21) it loses.
+    const int64_t p2 = j.n & -j.n;
+    if (p2 >= 16 && p2 <= 4096 && j.n / p2 <= 2048) n1 = p2;
     if (j.split) {
         // a power-of-two factor may use the compile-time kernels' larger cap
         auto fits = [&](int64_t f) { return f <= (is_pow2(f) ? (int64_t)max_single_pass(prec) : (int64_t)cap); };
