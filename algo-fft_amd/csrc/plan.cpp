@@ -435,8 +435,9 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
 const void* gen_kernel(int prec, int var, int dir);
 const void* copy_kernel(int prec);
 
-// Radix schedule for a 2/3/5/7-smooth n: radix-16 passes, the power-of-two
-// remainder, then 3s, 5s and 7s.  Empty if n has another prime factor.
+// Radix schedule for a 2/3/5/7-smooth n: radix-16 passes and the
+// power-of-two remainder, with the 3s, 5s and 7s first when there are any.
+// Empty if n has another prime factor.
 std::vector<int> smooth_schedule(int64_t n) {
     std::vector<int> r;
     int64_t m = n;
@@ -446,9 +447,22 @@ std::vector<int> smooth_schedule(int64_t n) {
     for (int q : {3, 5, 7})
         while (m % q == 0) { m /= q; odd.push_back(q); }
     if (m != 1) return {};
-    while (a >= 4) { r.push_back(16); a -= 4; }
-    if (a > 0) r.insert(r.begin(), 1 << a);
+    if (odd.empty()) {
+        // powers of two: the remainder radix first (the compile-time
+        // schedules' order, whose twiddle layout the ragged cores share)
+        while (a >= 4) { r.push_back(16); a -= 4; }
+        if (a > 0) r.insert(r.begin(), 1 << a);
+        return r;
+    }
+    // Other smooth lengths: odd radices first, then the power-of-two
+    // radices.  A radix-R pass writes its outputs R apart while Ns is small;
+    // odd strides spread over the LDS banks, strides of 8 or 16 points
+    // conflict, so the large power-of-two radices go where Ns is large.
+    // Measured on 512 MiB batches: N=6000 0.431 -> 0.386 ms, N=100000
+    // 1.044 -> 0.987 ms.
     for (int q : odd) r.push_back(q);
+    if (a % 4) r.push_back(1 << (a % 4));
+    for (int i = 0; i < a / 4; ++i) r.push_back(16);
     return r;
 }
 
