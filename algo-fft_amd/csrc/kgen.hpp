@@ -86,6 +86,69 @@ __device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS,
     }
 }
 
+// One in-place radix-R Stockham pass of a transform of N points with
+// N/R <= (16/R) * TT: thread t reads butterflies j = t, t+TT, ... (load(n)),
+// applies twiddles and the codelet, and after a barrier of the whole block
+// writes output point (j/Ns)*Ns*R + j%Ns + r*Ns (store(k, v)).  Every thread
+// of the block calls this; `active` threads do the work.
+template <int R, int DIR, typename T, class Load, class Store>
+__device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, int NS, unsigned NSM,
+                                           const cx<T>* __restrict__ tw, Load load, Store store) {
+    constexpr int MB = (16 + R - 1) / R;
+    const int NR = N / R;
+    cx<T> v[MB][R];
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            const int j = t + b * TT;
+            if (j < NR) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b][r] = load(j + r * NR);
+                const int k = fast_mod(j, NS, NSM);
+                if (NS > 1) {
+                    const cx<T>* w = tw + k * (R - 1);   // k-major table: k*(R-1) + r-1
+#pragma unroll
+                    for (int r = 1; r < R; ++r) v[b][r] = twmul<DIR>(v[b][r], w[r - 1]);
+                }
+                dft<R, DIR, 1>(v[b]);
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            const int j = t + b * TT;
+            if (j < NR) {
+                const int k = fast_mod(j, NS, NSM);
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) store(base + r * NS, v[b][r]);
+            }
+        }
+    }
+}
+
+template <int DIR, typename T, class Load, class Store>
+__device__ __forceinline__ void rt_pass_ip_any(int R, bool active, int t, int TT, int N, int NS, unsigned NSM,
+                                               const cx<T>* tw, Load load, Store store) {
+    switch (R) {
+    case 2: rt_pass_ip<2, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 3: rt_pass_ip<3, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 4: rt_pass_ip<4, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 5: rt_pass_ip<5, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 7: rt_pass_ip<7, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 8: rt_pass_ip<8, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 16: rt_pass_ip<16, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    default: {   // radix 1: the copy of a one-point transform
+        cx<T> v0 = mk<T>(0, 0);
+        if (active && t < N) v0 = load(t);
+        __syncthreads();
+        if (active && t < N) store(t, v0);
+    }
+    }
+}
+
 // VAR_ROW / VAR_COL / VAR_4A / VAR_4B as in kernels.hpp.  ROW-like variants
 // lay a transform out contiguously in LDS, COL variants interleave G columns.
 template <typename T, int VAR, int DIR>

@@ -131,69 +131,6 @@ __device__ __forceinline__ cx<T> rag_tw(const RJob& jb, long long i) {
     return cmul(lo[i & ((1LL << jb.shift) - 1)], hi[i >> jb.shift]);
 }
 
-// One in-place radix-R Stockham pass of a transform of N points with
-// N/R <= (16/R) * TT: thread t reads butterflies j = t, t+TT, ... (load(n)),
-// applies twiddles and the codelet, and after a barrier of the whole block
-// writes output point (j/Ns)*Ns*R + j%Ns + r*Ns (store(k, v)).  Every thread
-// of the block calls this; `active` threads do the work.
-template <int R, int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, int NS, unsigned NSM,
-                                           const cx<T>* __restrict__ tw, Load load, Store store) {
-    constexpr int MB = (16 + R - 1) / R;
-    const int NR = N / R;
-    cx<T> v[MB][R];
-    if (active) {
-#pragma unroll
-        for (int b = 0; b < MB; ++b) {
-            const int j = t + b * TT;
-            if (j < NR) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) v[b][r] = load(j + r * NR);
-                const int k = fast_mod(j, NS, NSM);
-                if (NS > 1) {
-                    const cx<T>* w = tw + k * (R - 1);   // k-major table: k*(R-1) + r-1
-#pragma unroll
-                    for (int r = 1; r < R; ++r) v[b][r] = twmul<DIR>(v[b][r], w[r - 1]);
-                }
-                dft<R, DIR, 1>(v[b]);
-            }
-        }
-    }
-    __syncthreads();
-    if (active) {
-#pragma unroll
-        for (int b = 0; b < MB; ++b) {
-            const int j = t + b * TT;
-            if (j < NR) {
-                const int k = fast_mod(j, NS, NSM);
-                const int base = (j - k) * R + k;
-#pragma unroll
-                for (int r = 0; r < R; ++r) store(base + r * NS, v[b][r]);
-            }
-        }
-    }
-}
-
-template <int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass_ip_any(int R, bool active, int t, int TT, int N, int NS, unsigned NSM,
-                                               const cx<T>* tw, Load load, Store store) {
-    switch (R) {
-    case 2: rt_pass_ip<2, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 3: rt_pass_ip<3, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 4: rt_pass_ip<4, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 5: rt_pass_ip<5, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 7: rt_pass_ip<7, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 8: rt_pass_ip<8, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 16: rt_pass_ip<16, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    default: {   // radix 1: the copy of a one-point transform
-        cx<T> v0 = mk<T>(0, 0);
-        if (active && t < N) v0 = load(t);
-        __syncthreads();
-        if (active && t < N) store(t, v0);
-    }
-    }
-}
-
 // `cnt` transforms of length L in LDS rows of pitch P (transform c at
 // buf + c*P), group c of TPT threads on transform c, all passes in place;
 // the last pass hands its outputs to sink(c, k, v), which may write the
@@ -439,8 +376,8 @@ __device__ void rag_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>
 // in [128, 2048])
 // run their column and row tasks on the radix-16 register cores of the
 // single-size kernels: 16 points per thread, L/16 threads per sub-transform,
-// G = RAG_PTS/L sub-transforms per task (the host cuts every such job into
-// tasks of exactly G), global loads straight into registers.  Each length L
+// G = RagS<T>::PTS/L sub-transforms per task (the host cuts every such job
+// into tasks of exactly G), global loads straight into registers.  Each length L
 // is its own kernel and launch: one kernel holding several lengths would need
 // the registers of all of them.  The twiddles come from the job's runtime
 // schedule, whose table has the layout of the matching Sched (the remainder
@@ -460,10 +397,23 @@ template <> struct P2Sched<4096> { using S = Sched<16, 16, 16>; };
 template <> struct P2Sched<8192> { using S = Sched<2, 16, 16, 16>; };
 template <> struct P2Sched<16384> { using S = Sched<4, 16, 16, 16>; };
 
-// LDS of the row tasks: G padded slabs (the column tasks need RAG_PTS).
+// Tasks of the power-of-two splits: PTS points (G = PTS/L sub-transforms of
+// L points) on PTS/16 threads.  8192 points for complex64 double the
+// adjacent columns a column task reads per row (256 bytes at L = 256)
+// against 4096-point tasks: job offsets in the packed layout are not
+// aligned, so each row segment also touches a neighbouring cache line.
+// Measured on a 4096-job mixed set: HBM traffic 1.59 -> 1.50 GB per
+// execution at the same graph time.
+template <typename T>
+struct RagS {
+    static constexpr int PTS = sizeof(T) == 4 ? 8192 : 4096;
+    static constexpr int THREADS = PTS / 16;
+};
+
+// LDS of the row tasks: G padded slabs (the column tasks need PTS points).
 template <typename T>
 struct RagP2 {
-    static constexpr int BUF = RAG_PTS + (RAG_PTS >> PadFor<T, 64>::value) + RAG_PTS / 64;
+    static constexpr int BUF = RagS<T>::PTS + (RagS<T>::PTS >> PadFor<T, 64>::value) + RagS<T>::PTS / 64;
 };
 
 // Column task: columns c0 .. c0+G-1 of x[m2*n1 + n2] (chirp * x zero-padded
@@ -472,7 +422,7 @@ struct RagP2 {
 template <typename T, int L, int SD, bool BLUE>
 __device__ __forceinline__ void p2_cols(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
     using S = typename P2Sched<L>::S;
-    constexpr int E = 16, TT = L / E, G = RAG_PTS / L;
+    constexpr int E = 16, TT = L / E, G = RagS<T>::PTS / L;
     const int tid = threadIdx.x, c = tid % G, t = tid / G;
     const long long col = tk.first + c, m2 = jb.m2;
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
@@ -519,7 +469,7 @@ __device__ __forceinline__ void p2_row_load(const cx<T>* __restrict__ row, int t
 template <typename T, int L, int SD>
 __device__ __forceinline__ void p2_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
     using S = typename P2Sched<L>::S;
-    constexpr int E = 16, TT = L / E, G = RAG_PTS / L, PADSH = PadFor<T, L>::value;
+    constexpr int E = 16, TT = L / E, G = RagS<T>::PTS / L, PADSH = PadFor<T, L>::value;
     const int tid = threadIdx.x, c = tid / TT, t = tid % TT;
     const int r0 = tk.first;
     cx<T> v[E];
@@ -656,7 +606,7 @@ __device__ __forceinline__ void p2_whole_blue(const RagArgs a, const RTask& tk, 
 template <typename T, int L>
 __device__ __forceinline__ void p2_cols2(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
     using S = typename P2Sched<L>::S;
-    constexpr int E = 16, TT = L / E, G = RAG_PTS / L;
+    constexpr int E = 16, TT = L / E, G = RagS<T>::PTS / L;
     const int tid = threadIdx.x, c = tid % G, t = tid / G;
     const long long p = tk.first + c, m2 = jb.m2;
     const cx<T>* __restrict__ sp = (const cx<T>*)a.scratch + jb.scr_off + p;
@@ -699,8 +649,8 @@ __global__ void __launch_bounds__(RAG_THREADS) k_rag_cols(RagArgs a) {
 }
 
 template <typename T, int L, int SD, bool BLUE>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_cols_p2(RagArgs a) {
-    __shared__ cx<T> lds[RAG_PTS];
+__global__ void __launch_bounds__(RagS<T>::THREADS) k_rag_cols_p2(RagArgs a) {
+    __shared__ cx<T> lds[RagS<T>::PTS];
     rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_cols<T, L, SD, BLUE>(a, jb, tk, lds); });
 }
 
@@ -711,20 +661,20 @@ __global__ void __launch_bounds__(RAG_THREADS) k_rag_rows(RagArgs a) {
 }
 
 template <typename T, int L, int SD>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_rows_p2(RagArgs a) {
+__global__ void __launch_bounds__(RagS<T>::THREADS) k_rag_rows_p2(RagArgs a) {
     __shared__ cx<T> lds[RagP2<T>::BUF];
     rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_rows<T, L, SD>(a, jb, tk, lds); });
 }
 
 template <typename T, int L>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_rows_blue(RagArgs a) {
+__global__ void __launch_bounds__(RagS<T>::THREADS) k_rag_rows_blue(RagArgs a) {
     __shared__ cx<T> lds[RagP2<T>::BUF];
     rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_rows_blue<T, L>(a, jb, tk, lds); });
 }
 
 template <typename T, int L>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_cols2(RagArgs a) {
-    __shared__ cx<T> lds[RAG_PTS];
+__global__ void __launch_bounds__(RagS<T>::THREADS) k_rag_cols2(RagArgs a) {
+    __shared__ cx<T> lds[RagS<T>::PTS];
     rag_queue(a, [=](const RJob& jb, const RTask& tk) { p2_cols2<T, L>(a, jb, tk, lds); });
 }
 

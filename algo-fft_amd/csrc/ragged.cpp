@@ -246,18 +246,20 @@ static const void* rag_kernel(int prec, const RagLaunch& l, int dir) {
 static int rag_threads(int prec, const RagLaunch& l) {
     int th = RAG_THREADS;
     if (whole_kind(l.kind)) (void)rag_whole_kernel(prec, l.kind, l.L, -1, &th);
+    else if (l.L) th = prec ? RagS<double>::THREADS : RagS<float>::THREADS;   // power-of-two split tasks
     return th;
 }
 
 // A split runs on the power-of-two cores when both lengths are powers of two
-// in the ranges those kernels are built for and the tasks hold RAG_PTS/L
-// sub-transforms each.
-static bool p2_split(int64_t m1, int64_t m2, int g_col, int g_row) {
+// in the ranges those kernels are built for, with tasks of pts/L
+// sub-transforms of exactly pts points (RagS<T>::PTS).
+static bool p2_split(int64_t m1, int64_t m2, int pts) {
     auto pow2 = [](int64_t L) { return L > 0 && (L & (L - 1)) == 0; };
     return pow2(m1) && pow2(m2) && m1 >= RAG_P2_COL_MIN && m1 <= RAG_P2_COL_MAX && m2 >= RAG_P2_ROW_MIN &&
-           m2 <= RAG_P2_ROW_MAX && g_col == RAG_PTS / m1 && g_row == RAG_PTS / m2 && m2 % g_col == 0 &&
-           m1 % g_row == 0;
+           m2 <= RAG_P2_ROW_MAX && m2 % (pts / m1) == 0 && m1 % (pts / m2) == 0;
 }
+
+static int p2_pts(int prec) { return prec ? RagS<double>::PTS : RagS<float>::PTS; }
 
 
 static int ragged_build(afft_ragged_s& r) {
@@ -314,11 +316,17 @@ static int ragged_build(afft_ragged_s& r) {
         } else {
             j.m1 = (int)m1;
             j.m2 = (int)m2;
-            // at most RAG_PTS points per task: every thread then holds at
-            // most 16/R butterflies of a radix-R pass
-            j.g_col = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m1, m2}));
-            j.g_row = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m2, m1}));
-            const bool rmaj = p2_split(m1, m2, j.g_col, j.g_row);
+            // power-of-two splits: p2_pts points per task on the compile-time
+            // cores; generic splits: at most RAG_PTS points per task, so that
+            // every thread holds at most 16/R butterflies of a radix-R pass
+            const bool rmaj = p2_split(m1, m2, p2_pts(r.prec));
+            if (rmaj) {
+                j.g_col = p2_pts(r.prec) / (int)m1;
+                j.g_row = p2_pts(r.prec) / (int)m2;
+            } else {
+                j.g_col = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m1, m2}));
+                j.g_row = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m2, m1}));
+            }
             if ((rc = schedule_into(r.tables, r.prec, m1, rmaj, j.s[0], scache))) return rc;
             if ((rc = schedule_into(r.tables, r.prec, m2, rmaj, j.s[1], scache))) return rc;
             auto tt = tcache.find(m);
@@ -365,8 +373,8 @@ static int ragged_build(afft_ragged_s& r) {
             continue;
         }
         // Power-of-two splits (every Bluestein job) run on the compile-time
-        // cores, which take exactly RAG_PTS/L sub-transforms per task.
-        const bool p2 = p2_split(m1, m2, j.g_col, j.g_row);
+        // cores, which take exactly p2_pts/L sub-transforms per task.
+        const bool p2 = p2_split(m1, m2, p2_pts(r.prec));
         if (blue && !p2) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no power-of-two split for Bluestein length");
         const int lc = blue ? RL_COLS_BLUE : p2 ? RL_COLS_P2 : RL_COLS;
         const int lr = blue ? RL_ROWS_BLUE : p2 ? RL_ROWS_P2 : RL_ROWS;
