@@ -117,6 +117,8 @@ def heuristic_choice(shape, cands):
     n, prec = shape["n"], shape["precision"]
     maxp = 16384 if prec == COMPLEX64 else 8192
     gcap = 8192 if prec == COMPLEX64 else 4096
+    # contiguous rows hold twice that in one pass (in place in LDS)
+    rcap = 2 * gcap if shape.get("stride", 1) <= 1 else gcap
     pow2 = n & (n - 1) == 0
     smooth = synth._is_smooth7(n)
     if pow2 and n <= maxp:
@@ -124,7 +126,7 @@ def heuristic_choice(shape, cands):
     elif pow2:
         lg = int(math.log2(n))
         want = (STRATEGY_SIXSTEP, 1 << TUNED_SPLIT.get((lg, prec), lg // 2))
-    elif smooth and n <= gcap:
+    elif smooth and n <= rcap:
         want = (STRATEGY_MIXED, 0)
     elif smooth:
         p2 = n & -n

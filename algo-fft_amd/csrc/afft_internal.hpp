@@ -143,6 +143,7 @@ int blue_tables(Plan& p, int64_t n, int64_t m, double inv_scale, BlueTables& bt,
 int split_twiddles(Plan& p, int prec, int64_t L, int* shift, const void** lo, const void** hi);
 int max_single_pass(int prec);
 int gen_cap(int prec);
+int gen_row_cap(int prec);          // contiguous rows: one in-place pass
 
 // ---- planning knowledge (tune.cpp, mlp.hip) --------------------------------------
 constexpr int NFEAT = 32;

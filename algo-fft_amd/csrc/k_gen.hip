@@ -23,6 +23,11 @@ const void* gen_kernel(int prec, int var, int dir) {
     return nullptr;
 }
 
+const void* gen_ip_kernel(int prec, int dir) {
+    if (prec == 0) return dir < 0 ? (const void*)&k_gen_ip<float, -1> : (const void*)&k_gen_ip<float, 1>;
+    return dir < 0 ? (const void*)&k_gen_ip<double, -1> : (const void*)&k_gen_ip<double, 1>;
+}
+
 const void* copy_kernel(int prec) {
     return prec == 0 ? (const void*)&k_copy<float> : (const void*)&k_copy<double>;
 }

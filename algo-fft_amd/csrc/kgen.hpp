@@ -230,4 +230,66 @@ __global__ void __launch_bounds__((VAR == VAR_ROW && sizeof(T) == 4 ? GEN_ROWTHR
     }
 }
 
+// Rows of one transform per block, every pass in place in ONE LDS buffer of
+// N points: thread t of TT = blockDim.x (a multiple of 64, TT >= N/16) reads
+// all its butterflies of a pass, the block passes a barrier, then it writes.
+// Half the LDS of the two-buffer form: complex64 rows up to 16384 points
+// (128 KiB) and complex128 up to 8192 fit one pass, and shorter rows leave
+// room for more blocks per CU.  The input is staged into LDS first (every
+// load of a thread in flight at once): loading it inside the first pass, as
+// the two-buffer kernel does, took the kernel to 128 VGPRs plus 325 spilled
+// at 1024 threads per block.  The last pass stores to global memory.
+template <typename T>
+struct GenIP {
+    static constexpr int THREADS = sizeof(T) == 4 ? 1024 : 512;
+    // waves per SIMD at the register use of k_gen_ip (128 VGPRs for
+    // complex64 at 1024 threads, about 200 for complex128 at 512)
+    static constexpr int WAVES_PER_SIMD = sizeof(T) == 4 ? 4 : 2;
+};
+
+template <typename T, int DIR>
+__global__ void __launch_bounds__(GenIP<T>::THREADS) k_gen_ip(GArgs g) {
+    extern __shared__ unsigned char smraw[];
+    cx<T>* sm = (cx<T>*)smraw;
+    const KArgs a = g.k;
+    const int N = g.n;
+    const int TT = blockDim.x;
+    const int t = threadIdx.x;
+    const long long cg = blockIdx.x;
+    const cx<T>* __restrict__ src = (const cx<T>*)a.in + (long long)blockIdx.y * a.in_bd + cg * a.in_cd;
+    cx<T>* __restrict__ dst = (cx<T>*)a.out + (long long)blockIdx.y * a.out_bd + cg * a.out_cd;
+    const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+    {
+        const bool ntl = a.nt & NT_LOAD;
+        cx<T> v[16];
+#pragma unroll
+        for (int h = 0; h < 16; ++h) {
+            const int n = t + h * TT;
+            v[h] = n < N ? (ntl ? ld_nt(src + n) : src[n]) : mk<T>(0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < 16; ++h)
+            if (t + h * TT < N) sm[t + h * TT] = v[h];
+    }
+    __syncthreads();
+    for (int p = 0; p < g.npass; ++p) {
+        const int R = g.rad[p];
+        const int NS = g.ns[p];
+        const unsigned NSM = g.nsm[p];
+        const cx<T>* twp = tw + g.twoff[p];
+        auto load_s = [&](int n) -> cx<T> { return sm[n]; };
+        auto store_s = [&](int k, cx<T> y) { sm[k] = y; };
+        auto store_o = [&](int k, cx<T> y) {
+            if (a.scale != 1.0) y = scl(y, (T)a.scale);
+            st_out(dst + k, y, a.nt);
+        };
+        if (p == g.npass - 1) {
+            rt_pass_ip_any<DIR>(R, true, t, TT, N, NS, NSM, twp, load_s, store_o);
+        } else {
+            rt_pass_ip_any<DIR>(R, true, t, TT, N, NS, NSM, twp, load_s, store_s);
+            __syncthreads();
+        }
+    }
+}
+
 }  // namespace afft

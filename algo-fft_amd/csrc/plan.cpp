@@ -433,6 +433,7 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
 
 // ---- runtime-size mixed radix ------------------------------------------------
 const void* gen_kernel(int prec, int var, int dir);
+const void* gen_ip_kernel(int prec, int dir);
 const void* copy_kernel(int prec);
 
 // Radix schedule for a 2/3/5/7-smooth n: radix-16 passes and the
@@ -467,6 +468,43 @@ std::vector<int> smooth_schedule(int64_t n) {
 }
 
 int gen_cap(int prec) { return prec ? 4096 : 8192; }
+int gen_row_cap(int prec) { return prec ? 8192 : 16384; }
+
+// Contiguous rows above this length run one transform per block with every
+// pass in place in one LDS buffer (k_gen_ip); shorter rows, and every other
+// layout, on the two-buffer kernel.  Up to here (40 KiB a row) two blocks of
+// the two-buffer kernel fit a CU's 160 KiB of LDS; above it one.  Measured
+// on 512 MiB batches: complex64 N=6000 0.387 -> 0.367 ms, N=8000 0.349 ->
+// 0.334 ms; complex128 N=3000 0.489 -> 0.328 ms, N=4000 0.451 -> 0.316 ms;
+// complex64 N=5000 stays on the two-buffer kernel (0.372 against 0.40 ms).
+static int gen_ip_min(int prec) { return prec ? 2560 : 5120; }
+
+// Threads of an in-place row block: the multiple of 64 from n/16 (each pass
+// holds at most 16 points of a thread) up to the kernel's maximum that keeps
+// the most threads resident per CU, counting blocks by LDS (160 KiB) and by
+// wave slots (waves per SIMD at the kernel's register use); on a tie the
+// smaller block.  E.g. complex64 N=8000 (64 KiB): 512 threads, two blocks
+// (0.334 ms) against 704 threads, one block (0.481 ms).
+static int gen_ip_threads(int64_t n, int prec) {
+    const int64_t es = prec ? 16 : 8;
+    const int maxt = prec ? GenIP<double>::THREADS : GenIP<float>::THREADS;
+    const int wps = prec ? GenIP<double>::WAVES_PER_SIMD : GenIP<float>::WAVES_PER_SIMD;
+    const int64_t lds_blocks = std::max<int64_t>(1, (int64_t)(160 << 10) / (n * es));
+    int best = maxt;
+    int64_t best_res = -1;
+    for (int tt = (int)(((n + 15) / 16 + 63) / 64 * 64); tt <= maxt; tt += 64) {
+        const int64_t wave_blocks = std::max(1, wps / ((tt / 64 + 3) / 4));
+        const int64_t res = std::min(lds_blocks, wave_blocks) * tt;
+        if (res > best_res) { best_res = res; best = tt; }
+    }
+    return best;
+}
+
+// Whether one runtime-radix launch holds a transform of n points on this
+// layout.
+static bool gen_single_fits(int prec, int64_t n, bool rows) {
+    return n <= gen_cap(prec) || (rows && n <= gen_row_cap(prec));
+}
 
 int64_t next_pow2(int64_t n) {
     int64_t m = 1;
@@ -490,6 +528,12 @@ struct GenCfg {
 // per block.
 static GenCfg gen_cfg(int64_t n, int prec, bool rows) {
     GenCfg c;
+    if (rows && n > gen_ip_min(prec)) {
+        c.tpt = c.threads = gen_ip_threads(n, prec);
+        c.g = 1;
+        c.lds = (int)(n * (prec ? 16 : 8));
+        return c;
+    }
     if (rows && n > 4096) c.tpt = prec == 0 ? GEN_ROWTHREADS : GEN_MAXTHREADS;
     else if (rows) c.tpt = (int)std::min<int64_t>(GEN_THREADS, next_pow2((n + 7) / 8));
     else c.tpt = (int)std::min<int64_t>(GEN_THREADS, next_pow2((n + 15) / 16));
@@ -585,10 +629,13 @@ static int emit_gen_single(Plan& p, int di, const AxisJob& j, std::string& algo)
             count = 1;
         }
     }
+    if (!gen_single_fits(prec, j.n, var == VAR_ROW))
+        return set_error(AFFT_ERR_NOT_IMPLEMENTED, "mixed radix: length too large for one pass on this layout");
     GArgs g{};
     int tpt, lds, threads, rc;
     if ((rc = gen_args(p, j.n, var == VAR_ROW, g, &tpt, &lds, &threads))) return rc;
-    const void* fn = gen_kernel(prec, var, dir);
+    const bool ip = var == VAR_ROW && j.n > gen_ip_min(prec);
+    const void* fn = ip ? gen_ip_kernel(prec, dir) : gen_kernel(prec, var, dir);
     if ((rc = prep_dynamic_lds(fn, lds))) return rc;
     g.k.count = count;
     g.k.in_cd = j.in_cd;
@@ -1063,6 +1110,7 @@ static bool needs_rows(const Plan& p, int64_t n, int strategy) {
 static int emit_axis_direct(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int prec = p.desc.precision;
     const bool smooth = !smooth_schedule(j.n).empty();
+    const bool rows = j.in_nd == 1 && j.out_nd == 1;
     if ((p.desc.flags & AFFT_FLAG_MFMA) &&
         (!is_pow2(j.n) || j.strategy == AFFT_STRATEGY_MIXED || j.strategy == AFFT_STRATEGY_BLUESTEIN))
         return set_error(AFFT_ERR_NOT_IMPLEMENTED,
@@ -1073,7 +1121,7 @@ static int emit_axis_direct(Plan& p, int di, const AxisJob& j, std::string& algo
         if (is_pow2(j.n) && j.n <= max_single_pass(prec)) return emit_single(p, di, j, algo);
         return set_error(AFFT_ERR_NOT_IMPLEMENTED, "Stockham strategy needs a power of two held in LDS");
     case AFFT_STRATEGY_MIXED:
-        if (smooth && j.n <= gen_cap(prec)) return emit_gen_single(p, di, j, algo);
+        if (smooth && gen_single_fits(prec, j.n, rows)) return emit_gen_single(p, di, j, algo);
         return set_error(AFFT_ERR_NOT_IMPLEMENTED, "mixed-radix strategy needs a 7-smooth size held in LDS");
     case AFFT_STRATEGY_SIXSTEP:
         if (is_pow2(j.n)) return emit_fourstep(p, di, j, algo);
@@ -1088,7 +1136,7 @@ static int emit_axis_direct(Plan& p, int di, const AxisJob& j, std::string& algo
     if (is_pow2(j.n) && j.n <= max_single_pass(prec)) return emit_single(p, di, j, algo);
     if (is_pow2(j.n)) return emit_fourstep(p, di, j, algo);
     if (!smooth_schedule(j.n).empty()) {
-        if (j.n <= gen_cap(prec)) return emit_gen_single(p, di, j, algo);
+        if (gen_single_fits(prec, j.n, rows)) return emit_gen_single(p, di, j, algo);
         return emit_gen_fourstep(p, di, j, algo);
     }
     return emit_bluestein(p, di, j, algo);

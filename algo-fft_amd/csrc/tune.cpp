@@ -30,7 +30,8 @@ std::vector<Candidate> plan_candidates(const afft_plan_desc& d) {
     const bool smooth = !smooth_schedule(n).empty();
     const int maxp = max_single_pass(prec), gcap = gen_cap(prec);
     if (is_pow2(n) && n <= maxp) out.push_back({AFFT_STRATEGY_STOCKHAM, 0});
-    if (smooth && n <= gcap) out.push_back({AFFT_STRATEGY_MIXED, 0});
+    const bool rows = d.istride <= 1 && d.ostride <= 1;
+    if (smooth && (n <= gcap || (rows && n <= gen_row_cap(prec)))) out.push_back({AFFT_STRATEGY_MIXED, 0});
     if (is_pow2(n)) {
         const int lg = ilog2i(n);
         for (int a = 4; a <= lg - 4; ++a) {
@@ -155,7 +156,7 @@ void plan_features(const afft_plan_desc& d, const Candidate& c, float* f) {
     f[19] = n1 ? (float)std::log2((double)n1) : 0.f;
     f[20] = n2 ? (float)std::log2((double)n2) : 0.f;
     f[21] = n <= max_single_pass(prec) ? 1.f : 0.f;
-    f[22] = n <= gen_cap(prec) ? 1.f : 0.f;
+    f[22] = (n <= gen_cap(prec) || (stride == 1 && n <= gen_row_cap(prec))) ? 1.f : 0.f;
     f[23] = (float)launches;
     f[24] = (float)std::log2(es * 2.0 * n * b * launches);
     f[25] = (split && is_pow2(n1)) ? 1.f : 0.f;

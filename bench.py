@@ -125,8 +125,8 @@ def sweep_leg(torch, A, ctx, stream, steps, x4096=None):
 
 def smooth_leg(torch, A, ctx, stream, steps):
     """2/3/5/7-smooth lengths (the mixed-radix path) at batch 2^26/N, 512 MiB
-    per buffer: N = 1000 and 3000 and 6000 in one pass, 12000 and 100000
-    as four-step."""
+    per buffer: N = 1000, 3000, 6000 and 12000 in one pass (6000 and 12000
+    with every pass in place in one LDS buffer), 100000 as four-step."""
     res = []
     g = torch.Generator(device="cuda:0").manual_seed(17)
     for n in (1000, 3000, 6000, 12000, 100000):

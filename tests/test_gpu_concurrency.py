@@ -67,7 +67,7 @@ def test_shared_plan_many_threads(ctx, n, threads):
 def test_concurrent_plan_creation(ctx):
     """Eight threads create and run plans of every kind at once
     (concurrent_test.go:149-211)."""
-    shapes = [(256,), (4096,), (1 << 16,), (3000,), (1009,), (64, 96), (8, 16, 32), (12000,)]
+    shapes = [(256,), (4096,), (1 << 16,), (3000,), (1009,), (64, 96), (8, 16, 32), (12000,), (24000,)]
     inputs = [synth.uniform_complex(7 + i, (2,) + s, np.complex64) for i, s in enumerate(shapes)]
     want = []
     for s, x in zip(shapes, inputs):
@@ -115,7 +115,7 @@ def test_concurrent_mixed_precisions(ctx):
         assert rel_l2(z, x) <= 2 * tol
 
 
-@pytest.mark.parametrize("n", [1 << 16, 12000, 8209])
+@pytest.mark.parametrize("n", [1 << 16, 24000, 8209])
 def test_one_plan_from_several_streams(ctx, n):
     """A plan with workspace (four-step, mixed four-step, Bluestein) enqueued
     on four streams in turn: each stream waits for the previous execution's
@@ -141,7 +141,7 @@ def test_one_plan_from_several_streams(ctx, n):
         assert torch.equal(y, r)
 
 
-@pytest.mark.parametrize("n", [16, 1024, 4096, 1 << 16, 1 << 20, 3000, 12000, 1009, 100003])
+@pytest.mark.parametrize("n", [16, 1024, 4096, 1 << 16, 1 << 20, 3000, 12000, 24000, 1009, 100003])
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_bit_identical_across_runs_streams_and_paths(ctx, n, dtype):
     """FuzzDeterministicForward (plan_fuzz_test.go:52-90), made strict: the

@@ -23,7 +23,7 @@ from helpers import C128_TOL, c64_tol, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-ROUTES = [1024, 4096, 65536, 3000, 12000, 1009, 8209]
+ROUTES = [1024, 4096, 65536, 3000, 12000, 24000, 1009, 8209]
 
 
 def _prec(dtype):

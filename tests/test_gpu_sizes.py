@@ -110,3 +110,40 @@ def test_interleaved_columns_mixed(ctx):
         p.forward(out, a)
     ref = O.fft_batch(np.ascontiguousarray(a.T).astype(np.complex128)).T
     assert rel_l2(out, ref) <= c64_tol(n)
+
+
+# Contiguous smooth rows held whole in one LDS buffer (one transform per
+# block, passes in place): complex64 5121..16384, complex128 2561..8192.
+IP_C64 = [5184, 6144, 7000, 9000, 10240, 13122, 15000, 16000, 16128]
+IP_C128 = [2592, 3000, 4000, 5000, 7168, 8000]
+
+
+@pytest.mark.parametrize("prec,n", [(A.COMPLEX64, n) for n in IP_C64] + [(A.COMPLEX128, n) for n in IP_C128])
+def test_in_place_rows_forward_inverse(ctx, prec, n):
+    dt = np.complex64 if prec == A.COMPLEX64 else np.complex128
+    tol = c64_tol(n) if prec == A.COMPLEX64 else C128_TOL
+    batch = 5
+    x = synth.uniform_complex(13 * n, (batch, n), dt)
+    got, algo = _plan_run(ctx, x)
+    assert algo == f"hip_mixed{n}r", algo
+    ref = O.fft_batch(x.astype(np.complex128))
+    assert rel_l2(got, ref) <= tol, (n, rel_l2(got, ref))
+    # the inverse carries 1/N; FLAG_NORMALIZE puts 1/N on the forward instead
+    back, _ = _plan_run(ctx, got, inverse=True)
+    ref_b = O.fft_batch(got.astype(np.complex128), inverse=True)
+    assert rel_l2(back, ref_b) <= tol, (n, rel_l2(back, ref_b))
+    assert rel_l2(back, x) <= 2 * tol
+    scaled, _ = _plan_run(ctx, x, flags=A.FLAG_NORMALIZE)
+    assert rel_l2(scaled, ref / n) <= tol
+
+
+@pytest.mark.parametrize("dims", [(3, 12000), (12000, 4), (2, 3, 7000)])
+def test_in_place_rows_in_nd_plans(ctx, dims):
+    # the contiguous axis runs as in-place rows; a long strided axis is
+    # gathered into rows first
+    x = synth.uniform_complex(17, dims, np.complex64)
+    with A.Plan.from_desc(ctx, dims, precision=A.COMPLEX64) as p:
+        out = np.empty_like(x)
+        p.forward(out, x)
+        algo = p.describe()
+    assert rel_l2(out, O.fftn(x.astype(np.complex128))) <= c64_tol(int(np.prod(dims))), algo
