@@ -188,14 +188,15 @@ struct cx2 {
 
 // Tiles whose contiguous global dimension is the sub-transform index c
 // (cnt <= G sub-transforms, G a power of two): lane c = tid mod G, line
-// l = tid / G + s * (RAG_THREADS / G) in step s; nsteps(L) steps cover L lines.
+// l = tid / G + s * (TH / G) in step s; nsteps(L) steps cover L lines.
+template <int TH = RAG_THREADS>
 struct ColLanes {
     int c, l0, step;
     __device__ explicit ColLanes(int G) {
         const int lg = __ffs(G) - 1;
         c = threadIdx.x & (G - 1);
         l0 = threadIdx.x >> lg;
-        step = RAG_THREADS >> lg;
+        step = TH >> lg;
     }
     __device__ int line(int s) const { return l0 + s * step; }
     __device__ int nsteps(int L) const { return (L + step - 1) / step; }
@@ -227,14 +228,14 @@ __device__ __forceinline__ int flat_steps(int total) { return (total + TH - 1) /
 
 // Contiguous run src[0 .. total) into LDS rows of pitch L + 1 (row c of the
 // run starts at src + c*L).
-template <typename T>
+template <typename T, int TH = RAG_THREADS>
 __device__ __forceinline__ void rag_stage_rows(const cx<T>* __restrict__ src, int total, int L, cx<T>* lds) {
     const int tid = threadIdx.x;
-    RowWalk<> w(L);
+    RowWalk<TH> w(L);
     rag_staged<sizeof(cx<T>)>(
-        flat_steps(total), [&](int s) { return src[min(tid + s * RAG_THREADS, total - 1)]; },
+        flat_steps<TH>(total), [&](int s) { return src[min(tid + s * TH, total - 1)]; },
         [&](int s, cx<T> v) {
-            const int i = tid + s * RAG_THREADS;
+            const int i = tid + s * TH;
             if (i < total) lds[i + w.c] = v;
             w.next();
         });
@@ -316,16 +317,33 @@ __device__ void rag_whole_blue(const RagArgs a, const RJob& jb, cx<T>* lds) {
         });
 }
 
+// Tasks of the generic splits: at most PTS points (G <= THREADS
+// sub-transforms of L points, G*L <= PTS) on THREADS threads, 16 points per
+// thread at most, in LDS rows of pitch L + 1.  Complex64 tasks of 8192
+// points on 512 threads held to 128 VGPRs (4 waves per SIMD, 7-12 VGPRs
+// spilled) read twice the adjacent columns per row of a 4096-point task.
+// Measured on the smooth split jobs of a 4096-job mixed set: 0.146 ms with
+// 4096-point tasks (161-164 VGPRs, 3 waves per SIMD), 0.149 ms with those
+// held to 128 VGPRs, 0.162 ms with 8192-point tasks at 165 VGPRs (one block
+// per CU), 0.131 ms with them at 128, 0.139 ms with 16384-point tasks.
+template <typename T>
+struct RagG {
+    static constexpr int PTS = sizeof(T) == 4 ? 8192 : RAG_PTS;
+    static constexpr int THREADS = PTS / 16;
+    static constexpr int BUF = PTS + THREADS;
+    static constexpr int MIN_WAVES = sizeof(T) == 4 ? 4 : 2;   // per SIMD
+};
+
 // Column task of a split job with generic passes: columns c0 .. c0+cnt-1 of
 // the m1 x m2 view x[m2*n1 + n2], FFT over n1 with sign SD, times
 // W_m^{SD * n2 * k1}, into S[k1*m2 + n2].
-template <typename T, int SD>
+template <typename T, int SD, int TH>
 __device__ void rag_cols(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
-    const int L = jb.m1, P = L + 1, G = jb.g_col, TPT = RAG_THREADS / G;
+    const int L = jb.m1, P = L + 1, G = jb.g_col, TPT = TH / G;
     const int cnt = tk.count, c0 = tk.first, m2 = jb.m2;
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
     cx<T>* __restrict__ S = (cx<T>*)a.scratch + jb.scr_off;
-    const ColLanes ln(G);
+    const ColLanes<TH> ln(G);
     const bool lane = ln.c < cnt;
     const int cc = c0 + min(ln.c, cnt - 1);
     const int ns = ln.nsteps(L);
@@ -348,16 +366,16 @@ __device__ void rag_cols(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>
 
 // Row task of a split job: rows r0 .. r0+cnt-1 of S (length m2), FFT over n2
 // with sign SD, output X[k1 + m1*k2] (scaled 1/m for the inverse).
-template <typename T, int SD>
+template <typename T, int SD, int TH>
 __device__ void rag_rows(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>* lds) {
-    const int L = jb.m2, P = L + 1, G = jb.g_row, TPT = RAG_THREADS / G;
+    const int L = jb.m2, P = L + 1, G = jb.g_row, TPT = TH / G;
     const int cnt = tk.count, r0 = tk.first, m1 = jb.m1;
-    rag_stage_rows<T>((const cx<T>*)a.scratch + jb.scr_off + (long long)r0 * L, cnt * L, L, lds);
+    rag_stage_rows<T, TH>((const cx<T>*)a.scratch + jb.scr_off + (long long)r0 * L, cnt * L, L, lds);
     auto sink = [&](int c, int k2, cx<T> v) { lds[c * P + k2] = v; };
     rag_passes<T, SD>(jb.s[1], L, P, cnt, TPT, lds, sink);
     const T sc = SD > 0 ? T(1) / T(jb.m) : T(1);
     cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off + r0;
-    const ColLanes ln(G);
+    const ColLanes<TH> ln(G);
     if (ln.c < cnt) {
         const int ns = ln.nsteps(L);
 #pragma unroll
@@ -643,9 +661,9 @@ __device__ __forceinline__ void rag_queue(const RagArgs a, Run run) {
 
 // The kernels of the launches (RL_* in launch order).
 template <typename T, int SD>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_cols(RagArgs a) {
-    __shared__ cx<T> lds[RAG_BUF];
-    rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_cols<T, SD>(a, jb, tk, lds); });
+__global__ void __launch_bounds__(RagG<T>::THREADS, RagG<T>::MIN_WAVES) k_rag_cols(RagArgs a) {
+    __shared__ cx<T> lds[RagG<T>::BUF];
+    rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_cols<T, SD, RagG<T>::THREADS>(a, jb, tk, lds); });
 }
 
 template <typename T, int L, int SD, bool BLUE>
@@ -655,9 +673,9 @@ __global__ void __launch_bounds__(RagS<T>::THREADS) k_rag_cols_p2(RagArgs a) {
 }
 
 template <typename T, int SD>
-__global__ void __launch_bounds__(RAG_THREADS) k_rag_rows(RagArgs a) {
-    __shared__ cx<T> lds[RAG_BUF];
-    rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_rows<T, SD>(a, jb, tk, lds); });
+__global__ void __launch_bounds__(RagG<T>::THREADS, RagG<T>::MIN_WAVES) k_rag_rows(RagArgs a) {
+    __shared__ cx<T> lds[RagG<T>::BUF];
+    rag_queue(a, [=](const RJob& jb, const RTask& tk) { rag_rows<T, SD, RagG<T>::THREADS>(a, jb, tk, lds); });
 }
 
 template <typename T, int L, int SD>

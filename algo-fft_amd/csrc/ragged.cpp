@@ -44,12 +44,16 @@ __global__ void k_rag_reset(int* counter) {
 
 // The launches form independent chains (the tasks of a job depend only on
 // earlier launches of the same chain): runtime-pass lengths, power-of-two
-// lengths, Bluestein jobs.  Each chain runs its split jobs' launches, then
-// its whole jobs' launches (whole jobs depend on nothing, so they fill the
-// chain's tail).  Measured on the 4096-job mixed set: 0.561 ms with the
-// whole-job launches on these three chains, against 0.604 ms with them on a
-// fourth chain of their own (best shares of each layout).
-constexpr int RAG_BRANCHES = 3;
+// lengths, split Bluestein jobs, whole Bluestein jobs.  The first two chains
+// run their split jobs' launches, then their whole jobs' launches (whole
+// jobs depend on nothing, so they fill the chain's tail).  Whole Bluestein
+// jobs are a chain of their own: four launches of few blocks each, which
+// at the tail of the split Bluestein chain (the longest) added about 95 us
+// of mostly idle device to it.  Measured on the 4096-job mixed set: 0.561
+// ms with them on the Bluestein chain, 0.515-0.517 ms on their own chain
+// whatever its enqueue order and share; 0.604 ms with every whole-job launch
+// on a chain of its own.
+constexpr int RAG_BRANCHES = 4;
 static int rag_branch(int kind) {
     switch (kind) {
     case RL_COLS:
@@ -58,7 +62,8 @@ static int rag_branch(int kind) {
     case RL_COLS_P2:
     case RL_ROWS_P2:
     case RL_WHOLE_P2: return 1;
-    default: return 2;   // RL_COLS_BLUE, RL_ROWS_BLUE, RL_COLS2, RL_WHOLE_BLUE
+    case RL_WHOLE_BLUE: return 3;
+    default: return 2;   // RL_COLS_BLUE, RL_ROWS_BLUE, RL_COLS2
     }
 }
 
@@ -247,6 +252,7 @@ static int rag_threads(int prec, const RagLaunch& l) {
     int th = RAG_THREADS;
     if (whole_kind(l.kind)) (void)rag_whole_kernel(prec, l.kind, l.L, -1, &th);
     else if (l.L) th = prec ? RagS<double>::THREADS : RagS<float>::THREADS;   // power-of-two split tasks
+    else if (l.kind == RL_COLS || l.kind == RL_ROWS) th = prec ? RagG<double>::THREADS : RagG<float>::THREADS;
     return th;
 }
 
@@ -317,15 +323,17 @@ static int ragged_build(afft_ragged_s& r) {
             j.m1 = (int)m1;
             j.m2 = (int)m2;
             // power-of-two splits: p2_pts points per task on the compile-time
-            // cores; generic splits: at most RAG_PTS points per task, so that
+            // cores; generic splits: at most RagG<T>::PTS points per task, so that
             // every thread holds at most 16/R butterflies of a radix-R pass
             const bool rmaj = p2_split(m1, m2, p2_pts(r.prec));
             if (rmaj) {
                 j.g_col = p2_pts(r.prec) / (int)m1;
                 j.g_row = p2_pts(r.prec) / (int)m2;
             } else {
-                j.g_col = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m1, m2}));
-                j.g_row = pow2_floor(std::min<int64_t>({RAG_THREADS, RAG_PTS / m2, m1}));
+                const int gth = r.prec ? RagG<double>::THREADS : RagG<float>::THREADS;
+                const int gpts = r.prec ? RagG<double>::PTS : RagG<float>::PTS;
+                j.g_col = pow2_floor(std::min<int64_t>({gth, gpts / m1, m2}));
+                j.g_row = pow2_floor(std::min<int64_t>({gth, gpts / m2, m1}));
             }
             if ((rc = schedule_into(r.tables, r.prec, m1, rmaj, j.s[0], scache))) return rc;
             if ((rc = schedule_into(r.tables, r.prec, m2, rmaj, j.s[1], scache))) return rc;
@@ -438,12 +446,13 @@ static int ragged_build(afft_ragged_s& r) {
     AFFT_HIP(hipMalloc(&r.counter, RAG_COUNTERS * sizeof(int)));
     // Each persistent launch gets a share of the device's resident blocks,
     // so that the concurrent branches run side by side instead of the first
-    // one filling every CU until its queue drains.  Weights 4:5:7 (runtime
-    // passes, power-of-two cores, Bluestein chain) over the branches
+    // one filling every CU until its queue drains.  Weights 4:5:7:2 (runtime
+    // passes, power-of-two cores, split and whole Bluestein jobs) over the branches
     // present, 1.7 device-fulls in all, at most one device-full per branch;
     // a set with one branch keeps the whole device.  Shares swept on the
-    // 4096-job mixed set: 0.561-0.61 ms over the settings tried.
-    static constexpr int weight[RAG_BRANCHES] = {4, 5, 7};
+    // 4096-job mixed set: 0.561-0.61 ms over the settings tried with three
+    // chains; the whole Bluestein chain's weight (1 to 4) changed nothing.
+    static constexpr int weight[RAG_BRANCHES] = {4, 5, 7, 2};
     bool present[RAG_BRANCHES] = {};
     for (const RagLaunch& l : r.launches) present[rag_branch(l.kind)] = true;
     int wsum = 0;
@@ -474,9 +483,13 @@ static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, 
     AFFT_HIP(hipEventRecord(r.fork, s));
     for (int b = 1; b < RAG_BRANCHES; ++b)
         if (used[b]) AFFT_HIP(hipStreamWaitEvent(r.branch[b - 1], r.fork, 0));
-    // Enqueue the longest chain first (Bluestein: three dependent passes),
-    // then the others; each branch keeps its own launch order.
-    static constexpr int order[RAG_BRANCHES] = {2, 1, 0};
+    // Enqueue the runtime-pass chain first (its generic kernels run at about
+    // half the per-point rate of the compile-time ones, so it finishes
+    // last), then split Bluestein, power-of-two, whole Bluestein; each branch
+    // keeps its own launch order.  Measured on the 4096-job mixed set:
+    // 0.508 ms against 0.515-0.518 ms with split Bluestein first; shares of
+    // 1.5 to 3.0 device-fulls 0.507-0.548 ms.
+    static constexpr int order[RAG_BRANCHES] = {0, 2, 1, 3};
     for (int ob = 0; ob < RAG_BRANCHES; ++ob)
     for (size_t p = 0; p < r.launches.size(); ++p) {
         const RagLaunch& l = r.launches[p];
