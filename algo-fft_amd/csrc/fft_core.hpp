@@ -66,7 +66,7 @@ struct LdsCol {
     __device__ __forceinline__ int operator()(int n) const { return n * TPB + c; }
 };
 
-template <typename T, class S, int E, int DIR, int p, class L>
+template <typename T, class S, int E, int DIR, int p, bool HOIST, class L>
 __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
                                           const L& li) {
     constexpr int N = S::N;
@@ -75,6 +75,30 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
     constexpr int NS = S::ns(p);
     constexpr int B = E / R;
     static_assert(E % R == 0, "every radix must divide the points per thread");
+    // r-major twiddle table: w^r of butterfly k at (r-1)*NS + k.  Radix 8
+    // and 16 load w^1, w^2, w^4 (, w^8) and form the other powers as at most
+    // three products: a quarter of the table traffic, at a cost of a few
+    // ulps on the composite powers.  The loads depend only on the thread's
+    // butterflies; with HOIST they are issued before the barrier that
+    // precedes the pass's LDS reads, so their latency overlaps the wait.
+    constexpr int NL = NS == 1 ? 1 : (R == 16 ? 4 : R == 8 ? 3 : R - 1);
+    cx<T> wl[B][NL];
+    auto load_tw = [&]() {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const cx<T>* __restrict__ w = tw + S::twoff(p) + (t + TT * b) % NS;
+            if constexpr (R == 16 || R == 8) {
+                wl[b][0] = w[0];
+                wl[b][1] = w[NS];
+                wl[b][2] = w[3 * NS];
+                if constexpr (R == 16) wl[b][3] = w[7 * NS];
+            } else {
+#pragma unroll
+                for (int r = 1; r < R; ++r) wl[b][r - 1] = w[(r - 1) * NS];
+            }
+        }
+    };
+    if constexpr (NS > 1 && HOIST) load_tw();
     if constexpr (p > 0) {
         __syncthreads();
 #pragma unroll
@@ -82,27 +106,21 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b + r * B] = lds[li(t + TT * b + r * (N / R))];
     }
+    if constexpr (NS > 1 && !HOIST) load_tw();
     if constexpr (NS > 1) {
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            const int j = t + TT * b;
-            const int k = j % NS;
-            // r-major table: w^r of butterfly k at (r-1)*NS + k
-            const cx<T>* __restrict__ w = tw + S::twoff(p) + k;
             if constexpr (R == 16 || R == 8) {
-                // Load w^1, w^2, w^4 (, w^8) and form the other powers as at
-                // most three products: a quarter of the table traffic, at a
-                // cost of a few ulps on the composite powers.
                 cx<T> ww[R];
-                ww[1] = w[0];
-                ww[2] = w[NS];
-                ww[4] = w[3 * NS];
+                ww[1] = wl[b][0];
+                ww[2] = wl[b][1];
+                ww[4] = wl[b][2];
                 ww[3] = cmul(ww[1], ww[2]);
                 ww[5] = cmul(ww[1], ww[4]);
                 ww[6] = cmul(ww[2], ww[4]);
                 ww[7] = cmul(ww[3], ww[4]);
                 if constexpr (R == 16) {
-                    ww[8] = w[7 * NS];
+                    ww[8] = wl[b][3];
 #pragma unroll
                     for (int r = 9; r < 16; ++r) ww[r] = cmul(ww[r - 8], ww[8]);
                 }
@@ -110,7 +128,7 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
                 for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], ww[r]);
             } else {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], w[(r - 1) * NS]);
+                for (int r = 1; r < R; ++r) v[b + r * B] = twmul<DIR>(v[b + r * B], wl[b][r - 1]);
             }
         }
     }
@@ -128,19 +146,20 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
     }
 }
 
-template <typename T, class S, int E, int DIR, class L, int... Ps>
+template <typename T, class S, int E, int DIR, bool HOIST, class L, int... Ps>
 __device__ __forceinline__ void core_all(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
                                          const L& li, std::integer_sequence<int, Ps...>) {
-    (core_pass<T, S, E, DIR, Ps>(v, lds, t, tw, li), ...);
+    (core_pass<T, S, E, DIR, Ps, HOIST>(v, lds, t, tw, li), ...);
 }
 
 // Full transform.  PRESYNC adds a barrier before the first LDS write, needed
-// when the LDS slab was read earlier in the same kernel.
-template <typename T, class S, int E, int DIR, bool PRESYNC = false, class L>
+// when the LDS slab was read earlier in the same kernel.  HOIST issues each
+// pass's twiddle loads before its barrier (see core_pass).
+template <typename T, class S, int E, int DIR, bool PRESYNC = false, bool HOIST = false, class L>
 __device__ __forceinline__ void core(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
                                      const L& li) {
     if constexpr (PRESYNC && S::P > 1) __syncthreads();
-    core_all<T, S, E, DIR>(v, lds, t, tw, li, std::make_integer_sequence<int, S::P>{});
+    core_all<T, S, E, DIR, HOIST>(v, lds, t, tw, li, std::make_integer_sequence<int, S::P>{});
 }
 
 }  // namespace afft

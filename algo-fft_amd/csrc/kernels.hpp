@@ -125,7 +125,13 @@ __global__ void __launch_bounds__((KCfg<T, S, E, TPB, VAR>::THREADS)) k_fft(KArg
 
     if constexpr (C::ROWL) {
         LdsRow<N, C::PADSH> li(c);
-        core<T, S, E, DIR>(v, lds, t, tw, li);
+        // one row per block of at most 256 threads: twiddle loads ahead of
+        // each pass's barrier.  Measured on 512 MiB batches: complex64
+        // N=4096 0.182 -> 0.177 ms, complex128 N=4096 0.189 -> 0.187 ms;
+        // with several rows per block (N=256), in the four-step passes and
+        // in the 512- and 1024-thread rows (complex128 8192, complex64
+        // 16384) it lost 1-2 %.
+        core<T, S, E, DIR, false, VAR == VAR_ROW && TPB == 1 && N / E <= 256>(v, lds, t, tw, li);
     } else {
         LdsCol<TPB> li(c);
         core<T, S, E, DIR>(v, lds, t, tw, li);

@@ -147,12 +147,37 @@ struct MfCfg {
     static_assert(T % 64 == 0 && T <= 1024, "whole waves, at most 1024 threads");
 };
 
+// Twiddles of radix-16 pass p (Ns > 1) for the four items of this lane:
+// w^r of butterfly jb % Ns for r = 4q + i, from the r-major table (entry
+// (r-1)*Ns + jb % Ns; r = 0, for q = 0 and i = 0, takes entry r = 1's slot
+// and is discarded, so no lane branches around a load).
+template <int N, int G, int p>
+__device__ __forceinline__ void mf_tw_load(const cx<float>* __restrict__ tw, cx<float> (&wt)[4][4]) {
+    using C = MfCfg<N, G>;
+    constexpr int NS = mf_ns(N, p), TWO = mf_twoff(N, p);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int q = lane >> 4, col = lane & 15;
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int item = wave + C::WAVES * it;
+        const int c = item / C::GROUPS, g = item - c * C::GROUPS;
+        const int jb = 16 * g + col;
+        const int tq = TWO + jb % NS + (4 * q - 1) * NS;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) wt[it][i] = tw[i > 0 ? tq + i * NS : (q > 0 ? tq : tq + NS)];
+    }
+}
+
+// Pass p.  Its twiddles arrive in wt, loaded by pass p - 1 before that
+// pass's closing barrier, so their latency overlaps the barrier wait; it
+// loads pass p + 1's the same way.  (Loaded where they are used, the
+// dependent global loads cost the N=4096 kernel about 20 % of its time.)
 template <int N, int G, int VAR, int DIR, int p>
 __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, unsigned* smax, const half8 (&Ah)[2],
                                         const half8 (&Al)[2], const cx<float>* __restrict__ in,
-                                        cx<float>* __restrict__ out, long long c0) {
+                                        cx<float>* __restrict__ out, long long c0, cx<float> (&wt)[4][4]) {
     using C = MfCfg<N, G>;
-    constexpr int R = mf_rad(N, p), NS = mf_ns(N, p), TWO = mf_twoff(N, p);
+    constexpr int R = mf_rad(N, p), NS = mf_ns(N, p);
     constexpr bool GIN = p == 0 && VAR != VAR_4A;          // first pass reads global rows
     constexpr bool GOUT = p == C::P - 1 && VAR == VAR_ROW; // last pass writes global rows
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -169,47 +194,56 @@ __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, unsigned
             const bool act = cg < a.count;
             // rows past the batch read the last row (no branch) and are zeroed
             const long long cl = act ? cg : a.count - 1;
+            // inputs n = n0 + i*NB; NB is a multiple of 16, so the padded LDS
+            // index of input i is the lane's base plus a constant
+            const int n0 = jb + 4 * q * C::NB;
+            const cx<float>* __restrict__ rp = in + (cl * a.in_cd + n0);
+            const int lb = c * C::PITCH + mf_pad(n0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int n = jb + (4 * q + i) * C::NB;
                 if constexpr (GIN) {
-                    const cx<float> v = in[cl * a.in_cd + n];
+                    const cx<float> v = rp[i * C::NB];
                     x[it][i] = act ? v : mk<float>(0, 0);
                 } else {
-                    x[it][i] = lds[c * C::PITCH + mf_pad(n)];
+                    x[it][i] = lds[lb + i * (C::NB + C::NB / 16)];
                 }
             }
         }
         if constexpr (GIN) {
             // this pass reads the input: the per-transform maximum (every
-            // lane of an item belongs to the item's transform)
-#pragma unroll
-            for (int it = 0; it < 4; ++it) {
+            // lane of an item belongs to the item's transform; with one
+            // transform per block, the four items of a wave share it)
+            if constexpr (G == 1) {
                 float m = 0.f;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) m = cmax(x[it][i], m);
+                for (int it = 0; it < 4; ++it)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) m = cmax(x[it][i], m);
                 m = wave_max<1>(m);
-                const int item = wave + C::WAVES * it;
-                if (lane == 0) atomicMax(&smax[item / C::GROUPS], __float_as_uint(m));
+                if (lane == 0) atomicMax(&smax[0], __float_as_uint(m));
+            } else {
+#pragma unroll
+                for (int it = 0; it < 4; ++it) {
+                    float m = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) m = cmax(x[it][i], m);
+                    m = wave_max<1>(m);
+                    const int item = wave + C::WAVES * it;
+                    if (lane == 0) atomicMax(&smax[item / C::GROUPS], __float_as_uint(m));
+                }
             }
             __syncthreads();
         }
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
             const int item = wave + C::WAVES * it;
-            const int c = item / C::GROUPS, g = item - c * C::GROUPS;
-            const int jb = 16 * g + col;
+            const int c = item / C::GROUPS;
             float down = 1.f;
             if constexpr (p == C::P0) down = pow2f(-mf_exp(smax[c]));
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = 4 * q + i;
-                if constexpr (NS > 1) {
-                    // r-major table; r = 0 (q = 0, i = 0) takes entry r = 1's
-                    // slot and discards it, so no lane branches around a load
-                    const cx<float> w = tw[TWO + (r > 0 ? r - 1 : 0) * NS + jb % NS];
-                    x[it][i] = r > 0 ? twmul<DIR>(x[it][i], w) : x[it][i];
-                }
+                if constexpr (NS > 1) x[it][i] = r > 0 ? twmul<DIR>(x[it][i], wt[it][i]) : x[it][i];
                 if constexpr (p == C::P0) x[it][i] = scl(x[it][i], down);
             }
             mf_dft16(Ah, Al, x[it], x[it]);
@@ -220,18 +254,27 @@ __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, unsigned
             const int item = wave + C::WAVES * it;
             const int c = item / C::GROUPS, g = item - c * C::GROUPS;
             const int jb = 16 * g + col;
-            const int base = (jb / NS) * NS * 16 + jb % NS;
-            const int ko[4] = {2 * q, 2 * q + 1, 8 + 2 * q, 9 + 2 * q};
+            // outputs idx = base + 2q*NS + {0, 1, 8, 9}*NS
+            const int b2 = (jb / NS) * NS * 16 + jb % NS + 2 * q * NS;
+            constexpr int KO[4] = {0, 1, 8, 9};
             const long long cg = c0 + c;
             float sc = 1.f;
             if constexpr (GOUT) sc = (float)a.scale * pow2f(mf_exp(smax[c]));
+            cx<float>* __restrict__ op = out + (cg * a.out_cd + b2);
+            // padded LDS index: with NS a multiple of 16 the padding of the
+            // constant part separates; with NS = 1 the outputs 16 jb + ko
+            // (ko < 16) pad to 17 jb + ko
+            const int lb = c * C::PITCH + (NS % 16 == 0 ? mf_pad(b2) : NS == 1 ? 17 * jb + 2 * q : 0);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int idx = base + ko[i] * NS;
                 if constexpr (GOUT) {
-                    if (cg < a.count) st_out(out + cg * a.out_cd + idx, scl(x[it][i], sc), a.nt);
+                    if (cg < a.count) st_out(op + KO[i] * NS, scl(x[it][i], sc), a.nt);
+                } else if constexpr (NS % 16 == 0) {
+                    lds[lb + KO[i] * (NS + NS / 16)] = x[it][i];
+                } else if constexpr (NS == 1) {
+                    lds[lb + KO[i]] = x[it][i];
                 } else {
-                    lds[c * C::PITCH + mf_pad(idx)] = x[it][i];
+                    lds[c * C::PITCH + mf_pad(b2 + KO[i] * NS)] = x[it][i];
                 }
             }
         }
@@ -272,6 +315,9 @@ __device__ __forceinline__ void mf_pass(const KArgs& a, cx<float>* lds, unsigned
             for (int r = 0; r < R; ++r) lds[c * C::PITCH + mf_pad(R * j + r)] = v[i][r];
         }
     }
+    if constexpr (p + 1 < C::P) {
+        if constexpr (mf_rad(N, p + 1) == 16 && mf_ns(N, p + 1) > 1) mf_tw_load<N, G, p + 1>(tw, wt);
+    }
     if constexpr (!GOUT) __syncthreads();
 }
 
@@ -279,7 +325,8 @@ template <int N, int G, int VAR, int DIR, int... Ps>
 __device__ __forceinline__ void mf_passes(const KArgs& a, cx<float>* lds, unsigned* smax, const half8 (&Ah)[2],
                                           const half8 (&Al)[2], const cx<float>* in, cx<float>* out, long long c0,
                                           std::integer_sequence<int, Ps...>) {
-    (mf_pass<N, G, VAR, DIR, Ps>(a, lds, smax, Ah, Al, in, out, c0), ...);
+    cx<float> wt[4][4];
+    (mf_pass<N, G, VAR, DIR, Ps>(a, lds, smax, Ah, Al, in, out, c0, wt), ...);
 }
 
 template <int N, int G, int VAR, int DIR>
