@@ -63,10 +63,17 @@ def traffic_bytes(n, batch):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
-def _time_plan(torch, plan, A, stream, y, x, steps):
+def _time_plan(torch, plan, A, stream, y, x, steps, warm_s=0.1):
+    """Mean time of `steps` back-to-back executions, after `warm_s` seconds of
+    untimed ones: the host work between legs (plan creation, accuracy checks
+    on the CPU) idles the GPU long enough for its clocks to drop."""
     sh = stream.cuda_stream
-    plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)
-    torch.cuda.synchronize()
+    t0 = time.time()
+    while True:
+        plan.exec_device(A.FORWARD, y.data_ptr(), x.data_ptr(), sh)
+        torch.cuda.synchronize()
+        if time.time() - t0 >= warm_s:
+            break
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(steps):
