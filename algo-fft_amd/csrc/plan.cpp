@@ -470,40 +470,32 @@ std::vector<int> smooth_schedule(int64_t n) {
 int gen_cap(int prec) { return prec ? 4096 : 8192; }
 int gen_row_cap(int prec) { return prec ? 8192 : 16384; }
 
-// Contiguous rows above this length run one transform per block with every
-// pass in place in one LDS buffer (k_gen_ip); shorter rows, and every other
-// layout, on the two-buffer kernel.  Up to here (40 KiB a row) two blocks of
-// the two-buffer kernel fit a CU's 160 KiB of LDS; above it one.  Measured
-// on 512 MiB batches: complex64 N=6000 0.387 -> 0.367 ms, N=8000 0.349 ->
-// 0.334 ms; complex128 N=3000 0.489 -> 0.328 ms, N=4000 0.451 -> 0.316 ms;
-// complex64 N=5000 stays on the two-buffer kernel (0.372 against 0.40 ms).
-static int gen_ip_min(int prec) { return prec ? 2560 : 5120; }
+// Threads resident per CU for blocks of `threads` threads and `lds` bytes
+// of LDS, at `wps` waves per SIMD (the kernel's register use): blocks
+// counted by LDS (160 KiB) and by wave slots.
+static int64_t gen_resident(int threads, int64_t lds, int wps) {
+    const int64_t lds_blocks = std::max<int64_t>(1, (int64_t)(160 << 10) / std::max<int64_t>(1, lds));
+    const int64_t wave_blocks = std::max(1, wps / ((threads / 64 + 3) / 4));
+    return std::min(lds_blocks, wave_blocks) * threads;
+}
 
 // Threads of an in-place row block: the multiple of 64 from n/16 (each pass
 // holds at most 16 points of a thread) up to the kernel's maximum that keeps
-// the most threads resident per CU, counting blocks by LDS (160 KiB) and by
-// wave slots (waves per SIMD at the kernel's register use); on a tie the
-// smaller block.  E.g. complex64 N=8000 (64 KiB): 512 threads, two blocks
-// (0.334 ms) against 704 threads, one block (0.481 ms).
-static int gen_ip_threads(int64_t n, int prec) {
+// the most threads resident per CU; on a tie the smaller block.  E.g.
+// complex64 N=8000 (64 KiB): 512 threads, two blocks (0.334 ms) against 704
+// threads, one block (0.481 ms).
+static int gen_ip_threads(int64_t n, int prec, int64_t* resident = nullptr) {
     const int64_t es = prec ? 16 : 8;
     const int maxt = prec ? GenIP<double>::THREADS : GenIP<float>::THREADS;
     const int wps = prec ? GenIP<double>::WAVES_PER_SIMD : GenIP<float>::WAVES_PER_SIMD;
-    const int64_t lds_blocks = std::max<int64_t>(1, (int64_t)(160 << 10) / (n * es));
     int best = maxt;
     int64_t best_res = -1;
     for (int tt = (int)(((n + 15) / 16 + 63) / 64 * 64); tt <= maxt; tt += 64) {
-        const int64_t wave_blocks = std::max(1, wps / ((tt / 64 + 3) / 4));
-        const int64_t res = std::min(lds_blocks, wave_blocks) * tt;
+        const int64_t res = gen_resident(tt, n * es, wps);
         if (res > best_res) { best_res = res; best = tt; }
     }
+    if (resident) *resident = best_res;
     return best;
-}
-
-// Whether one runtime-radix launch holds a transform of n points on this
-// layout.
-static bool gen_single_fits(int prec, int64_t n, bool rows) {
-    return n <= gen_cap(prec) || (rows && n <= gen_row_cap(prec));
 }
 
 int64_t next_pow2(int64_t n) {
@@ -526,14 +518,8 @@ struct GenCfg {
 // batches: N=1000 rows 0.37 -> 0.26 ms, N=6000 rows 0.65 -> 0.50 ms (512
 // threads) -> 0.45 ms (1024 threads); column passes lose with fewer columns
 // per block.
-static GenCfg gen_cfg(int64_t n, int prec, bool rows) {
+static GenCfg gen_cfg_2b(int64_t n, int prec, bool rows) {
     GenCfg c;
-    if (rows && n > gen_ip_min(prec)) {
-        c.tpt = c.threads = gen_ip_threads(n, prec);
-        c.g = 1;
-        c.lds = (int)(n * (prec ? 16 : 8));
-        return c;
-    }
     if (rows && n > 4096) c.tpt = prec == 0 ? GEN_ROWTHREADS : GEN_MAXTHREADS;
     else if (rows) c.tpt = (int)std::min<int64_t>(GEN_THREADS, next_pow2((n + 7) / 8));
     else c.tpt = (int)std::min<int64_t>(GEN_THREADS, next_pow2((n + 15) / 16));
@@ -541,6 +527,46 @@ static GenCfg gen_cfg(int64_t n, int prec, bool rows) {
     c.g = c.threads / c.tpt;
     c.lds = (int)(2 * c.g * n * (prec ? 16 : 8));
     return c;
+}
+
+// Contiguous rows run one transform per block with every pass in place in
+// one LDS buffer (k_gen_ip) above the two-buffer kernel's reach (8192
+// complex64 / 4096 complex128 points), and below it where that keeps at
+// least twice the threads resident per CU of the two-buffer layout, whose
+// rows hold 2n points (k_gen rows take 94-97 VGPRs for complex64, 4 waves
+// per SIMD, and 165 for complex128, 3), or as many where the two-buffer
+// layout runs a single block per CU.  Measured on 512 MiB batches
+// (two-buffer -> in place): complex64 N=3500 0.416 -> 0.313 ms, N=4000
+// 0.409 -> 0.323 ms (two blocks of 256 threads against four), N=4500 0.445
+// -> 0.417 ms (one block of 1024 threads against two of 512), N=6000 0.387
+// -> 0.365 ms, N=8000 0.349 -> 0.334 ms; complex128 N=3000 0.489 -> 0.328
+// ms.  Kept on the two-buffer kernel: complex64 N=3000 (three blocks of 256
+// against four: 0.341 against 0.350 ms), N=2000 (0.265 against 0.396 ms),
+// complex128 N=2500 (0.290 against 0.335 ms).
+static bool gen_use_ip(int64_t n, int prec) {
+    if (n > gen_cap(prec)) return true;
+    const GenCfg c = gen_cfg_2b(n, prec, true);
+    int64_t res_ip = 0;
+    gen_ip_threads(n, prec, &res_ip);
+    const int64_t res_2b = gen_resident(c.threads, c.lds, prec ? 3 : 4);
+    return res_ip >= 2 * res_2b || (res_ip >= res_2b && res_2b == c.threads);
+}
+
+// Whether one runtime-radix launch holds a transform of n points on this
+// layout.
+static bool gen_single_fits(int prec, int64_t n, bool rows) {
+    return n <= gen_cap(prec) || (rows && n <= gen_row_cap(prec));
+}
+
+static GenCfg gen_cfg(int64_t n, int prec, bool rows) {
+    if (rows && gen_use_ip(n, prec)) {
+        GenCfg c;
+        c.tpt = c.threads = gen_ip_threads(n, prec);
+        c.g = 1;
+        c.lds = (int)(n * (prec ? 16 : 8));
+        return c;
+    }
+    return gen_cfg_2b(n, prec, rows);
 }
 
 static int gen_args(Plan& p, int64_t n, bool rows, GArgs& g, int* tpt, int* lds, int* threads) {
@@ -634,7 +660,7 @@ static int emit_gen_single(Plan& p, int di, const AxisJob& j, std::string& algo)
     GArgs g{};
     int tpt, lds, threads, rc;
     if ((rc = gen_args(p, j.n, var == VAR_ROW, g, &tpt, &lds, &threads))) return rc;
-    const bool ip = var == VAR_ROW && j.n > gen_ip_min(prec);
+    const bool ip = var == VAR_ROW && gen_use_ip(j.n, prec);
     const void* fn = ip ? gen_ip_kernel(prec, dir) : gen_kernel(prec, var, dir);
     if ((rc = prep_dynamic_lds(fn, lds))) return rc;
     g.k.count = count;
