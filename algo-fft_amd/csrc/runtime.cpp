@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <thread>
+#include <vector>
 
 #include "afft_internal.hpp"
 
@@ -405,14 +407,16 @@ int afft_plan_io_len(afft_plan p, int direction, int64_t* in_len, int64_t* out_l
 
 // ---- host-slice execution ------------------------------------------------------
 // A batched 1-D plan whose host slices exceed kPipeMinBytes runs in chunks of
-// whole transforms on two streams: chunk k's upload, transform and download
-// are queued on stream k & 1, so one chunk's upload overlaps the previous
-// chunk's transform and download.  Pinned caller memory (afft_host_alloc or
-// any hipHostMalloc block) is transferred directly; pageable memory goes
-// through two pinned staging slots per direction, copied on the host while
-// the other slot's transfers run.
+// whole transforms through kPipeSlots device slots on three streams: one
+// queue of uploads, one of transforms, one of downloads, ordered by events.
+// Chunk k's upload overlaps chunk k-1's transform and chunk k-2's download,
+// so the host-to-device and device-to-host DMA engines both stay busy (the
+// link is full duplex).  Pinned caller memory (afft_host_alloc or any
+// hipHostMalloc block) is transferred directly; pageable memory goes through
+// pinned staging slots, copied on the host while other slots' transfers run.
 constexpr size_t kPipeMinBytes = size_t(64) << 20;
 constexpr size_t kPipeChunkBytes = size_t(16) << 20;
+constexpr int kPipeSlots = 3;
 
 }  // extern "C"
 
@@ -420,35 +424,60 @@ namespace afft {
 struct HostPipe {
     int dir_index = -1;
     int64_t chunk = 0;                 // transforms per full chunk
-    afft_plan full[2] = {nullptr, nullptr};
+    afft_plan full = nullptr;          // plan of `chunk` transforms
     afft_plan tail = nullptr;
     int64_t tail_batch = 0;
-    void* hin[2] = {nullptr, nullptr};
-    void* hout[2] = {nullptr, nullptr};
-    void* din[2] = {nullptr, nullptr};
-    void* dout[2] = {nullptr, nullptr};
-    hipStream_t s[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    void* hin[kPipeSlots] = {};
+    void* hout[kPipeSlots] = {};
+    void* din[kPipeSlots] = {};
+    void* dout[kPipeSlots] = {};
+    hipStream_t up = nullptr, comp = nullptr, down = nullptr;
+    hipEvent_t ev_up[kPipeSlots] = {}, ev_comp[kPipeSlots] = {}, ev_down[kPipeSlots] = {};
 };
 
 void destroy_host_pipe(HostPipe* h) {
     if (!h) return;
-    for (int i = 0; i < 2; ++i) {
-        if (h->s[i]) (void)hipStreamSynchronize(h->s[i]);
-        if (h->full[i]) afft_plan_destroy(h->full[i]);
+    for (hipStream_t st : {h->up, h->comp, h->down})
+        if (st) (void)hipStreamSynchronize(st);
+    for (int i = 0; i < kPipeSlots; ++i) {
         if (h->hin[i]) (void)hipHostFree(h->hin[i]);
         if (h->hout[i]) (void)hipHostFree(h->hout[i]);
         if (h->din[i]) (void)hipFree(h->din[i]);
         if (h->dout[i]) (void)hipFree(h->dout[i]);
-        if (h->ev[i]) (void)hipEventDestroy(h->ev[i]);
-        if (h->s[i]) (void)hipStreamDestroy(h->s[i]);
+        for (hipEvent_t ev : {h->ev_up[i], h->ev_comp[i], h->ev_down[i]})
+            if (ev) (void)hipEventDestroy(ev);
     }
+    for (hipStream_t st : {h->up, h->comp, h->down})
+        if (st) (void)hipStreamDestroy(st);
+    if (h->full) afft_plan_destroy(h->full);
     if (h->tail) afft_plan_destroy(h->tail);
     delete h;
 }
 }  // namespace afft
 
 extern "C" {
+
+// Host copy between a caller's pageable slice and a pinned staging slot.
+// One core copies about 15-30 GB/s, below what the link moves each way, so
+// copies of 8 MiB and more are split over four threads.
+static void host_copy(void* dst, const void* src, size_t n) {
+    constexpr size_t kSplitMin = size_t(8) << 20;
+    constexpr int kThreads = 4;
+    if (n < kSplitMin) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t part = (n / kThreads + 4095) & ~size_t(4095);
+    std::vector<std::thread> ts;
+    for (int t = 1; t < kThreads; ++t) {
+        const size_t off = part * t;
+        if (off >= n) break;
+        const size_t len = std::min(part, n - off);
+        ts.emplace_back([=] { std::memcpy((char*)dst + off, (const char*)src + off, len); });
+    }
+    std::memcpy(dst, src, std::min(part, n));
+    for (auto& t : ts) t.join();
+}
 
 static bool host_pinned(const void* ptr) {
     hipPointerAttribute_t at{};
@@ -490,18 +519,19 @@ static int host_exec_pipelined(afft_plan p, int dir, void* dst, const void* src)
         h->dir_index = di;
         h->chunk = std::max<int64_t>(1, (int64_t)(kPipeChunkBytes / std::max(ipt, opt)));
         h->chunk = std::min<int64_t>(h->chunk, (B + 3) / 4);
-        int rc = AFFT_OK;
+        int rc = child_plan(p, h->chunk, &h->full);
         hipError_t e = hipSuccess;
-        for (int i = 0; i < 2 && !rc && e == hipSuccess; ++i) {
-            rc = child_plan(p, h->chunk, &h->full[i]);
-            if (rc) break;
+        for (int i = 0; i < kPipeSlots && !rc && e == hipSuccess; ++i) {
             e = hipHostMalloc(&h->hin[i], h->chunk * ipt, hipHostMallocDefault);
             if (e == hipSuccess) e = hipHostMalloc(&h->hout[i], h->chunk * opt, hipHostMallocDefault);
             if (e == hipSuccess) e = hipMalloc(&h->din[i], h->chunk * ipt);
             if (e == hipSuccess) e = hipMalloc(&h->dout[i], h->chunk * opt);
-            if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->s[i], hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev[i], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_up[i], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_comp[i], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_down[i], hipEventDisableTiming);
         }
+        for (hipStream_t* st : {&h->up, &h->comp, &h->down})
+            if (!rc && e == hipSuccess) e = hipStreamCreateWithFlags(st, hipStreamNonBlocking);
         if (!rc && e != hipSuccess) rc = set_error(AFFT_ERR_DEVICE, std::string("host pipeline: ") + hipGetErrorString(e));
         if (rc) {
             destroy_host_pipe(h);
@@ -521,39 +551,56 @@ static int host_exec_pipelined(afft_plan p, int dir, void* dst, const void* src)
     const bool pin_in = host_pinned(src), pin_out = host_pinned(dst);
     const int64_t nch = (B + h.chunk - 1) / h.chunk;
     auto count_of = [&](int64_t k) { return std::min<int64_t>(h.chunk, B - k * h.chunk); };
-    // chunk k's output leaves the pinned slot once its event has fired
+    // chunk k's output leaves its pinned slot once its download has finished
     auto drain = [&](int64_t k) -> int {
-        const int slot = (int)(k & 1);
-        AFFT_HIP(hipEventSynchronize(h.ev[slot]));
-        if (!pin_out) std::memcpy((char*)dst + (size_t)(k * h.chunk) * opt, h.hout[slot], (size_t)count_of(k) * opt);
+        const int slot = (int)(k % kPipeSlots);
+        AFFT_HIP(hipEventSynchronize(h.ev_down[slot]));
+        if (!pin_out) host_copy((char*)dst + (size_t)(k * h.chunk) * opt, h.hout[slot], (size_t)count_of(k) * opt);
         return AFFT_OK;
     };
     for (int64_t k = 0; k < nch; ++k) {
-        const int slot = (int)(k & 1);
-        if (k >= 2) {
-            int rc = drain(k - 2);
+        const int slot = (int)(k % kPipeSlots);
+        const bool reuse = k >= kPipeSlots;   // the slot held chunk k - kPipeSlots
+        if (reuse && !pin_out) {
+            int rc = drain(k - kPipeSlots);
             if (rc) return rc;
         }
         const int64_t nb = count_of(k);
-        afft_plan cp = nb == h.chunk ? h.full[slot] : h.tail;
+        afft_plan cp = nb == h.chunk ? h.full : h.tail;
         const char* sp = (const char*)src + (size_t)(k * h.chunk) * ipt;
         char* dp = (char*)dst + (size_t)(k * h.chunk) * opt;
-        const void* up = sp;
+        const void* upsrc = sp;
         if (!pin_in) {
-            std::memcpy(h.hin[slot], sp, (size_t)nb * ipt);
-            up = h.hin[slot];
+            // the staging slot's previous upload must have finished reading it
+            if (reuse) AFFT_HIP(hipEventSynchronize(h.ev_up[slot]));
+            host_copy(h.hin[slot], sp, (size_t)nb * ipt);
+            upsrc = h.hin[slot];
         }
-        AFFT_HIP(hipMemcpyAsync(h.din[slot], up, (size_t)nb * ipt, hipMemcpyHostToDevice, h.s[slot]));
-        int rc = exec_plan(*cp, dir, h.dout[slot], h.din[slot], h.s[slot]);
+        // upload into din[slot] once the slot's previous transform has read it
+        if (reuse) AFFT_HIP(hipStreamWaitEvent(h.up, h.ev_comp[slot], 0));
+        AFFT_HIP(hipMemcpyAsync(h.din[slot], upsrc, (size_t)nb * ipt, hipMemcpyHostToDevice, h.up));
+        AFFT_HIP(hipEventRecord(h.ev_up[slot], h.up));
+        // transform into dout[slot] once uploaded and once the slot's previous
+        // download has read dout[slot]
+        AFFT_HIP(hipStreamWaitEvent(h.comp, h.ev_up[slot], 0));
+        if (reuse) AFFT_HIP(hipStreamWaitEvent(h.comp, h.ev_down[slot], 0));
+        int rc = exec_plan(*cp, dir, h.dout[slot], h.din[slot], h.comp);
         if (rc) return rc;
+        AFFT_HIP(hipEventRecord(h.ev_comp[slot], h.comp));
+        AFFT_HIP(hipStreamWaitEvent(h.down, h.ev_comp[slot], 0));
         AFFT_HIP(hipMemcpyAsync(pin_out ? (void*)dp : h.hout[slot], h.dout[slot], (size_t)nb * opt,
-                                hipMemcpyDeviceToHost, h.s[slot]));
-        AFFT_HIP(hipEventRecord(h.ev[slot], h.s[slot]));
+                                hipMemcpyDeviceToHost, h.down));
+        AFFT_HIP(hipEventRecord(h.ev_down[slot], h.down));
     }
-    for (int64_t k = std::max<int64_t>(0, nch - 2); k < nch; ++k) {
-        int rc = drain(k);
-        if (rc) return rc;
+    if (pin_out) {
+        AFFT_HIP(hipStreamSynchronize(h.down));
+    } else {
+        for (int64_t k = std::max<int64_t>(0, nch - kPipeSlots); k < nch; ++k) {
+            int rc = drain(k);
+            if (rc) return rc;
+        }
     }
+    AFFT_HIP(hipStreamSynchronize(h.up));
     clear_error();
     return AFFT_OK;
 }
