@@ -505,7 +505,23 @@ static bool pipe_eligible(afft_plan p, int di) {
     return std::max(ib, ob) >= kPipeMinBytes;
 }
 
+static int host_exec_pipelined_run(afft_plan p, int dir, void* dst, const void* src);
+
+// On an error part of the way through, the transfers already queued still
+// read the caller's source and write its destination: wait for them before
+// returning, so the caller's slices are not touched after the call.
 static int host_exec_pipelined(afft_plan p, int dir, void* dst, const void* src) {
+    const int rc = host_exec_pipelined_run(p, dir, dst, src);
+    if (rc != AFFT_OK && p->pipe) {
+        const std::string keep = afft_last_error();
+        for (hipStream_t st : {p->pipe->up, p->pipe->comp, p->pipe->down})
+            if (st) (void)hipStreamSynchronize(st);
+        set_error(rc, keep);
+    }
+    return rc;
+}
+
+static int host_exec_pipelined_run(afft_plan p, int dir, void* dst, const void* src) {
     const int di = dir == AFFT_INVERSE ? 1 : 0;
     const int64_t B = p->desc.batch;
     const size_t ipt = (size_t)(p->in_elems[di] / B) * p->in_esize[di];    // bytes per transform
