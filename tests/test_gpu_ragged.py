@@ -176,3 +176,22 @@ def test_ragged_whole_jobs_up_to_the_lds_cap(ctx, dtype):
             s.synchronize()
             print("cap", cap, "inverse" if inverse else "forward", "worst err/tol",
                   _check(sizes, offs, x, yt.cpu().numpy(), dtype, inverse=inverse))
+
+
+@pytest.mark.parametrize("sizes", [[1], [4096], [12000], [1 << 18], [100003], [5003] * 7, [3, 65536, 3]],
+                         ids=["one-point", "pow2-whole", "smooth-whole", "pow2-split", "blue-split",
+                              "blue-whole-only", "two-chains"])
+def test_ragged_sets_with_one_or_two_chains(ctx, sizes):
+    # sets whose jobs all fall into one or two of the concurrent chains: the
+    # launches of the chains present share the device, the others are absent
+    with A.Ragged(ctx, sizes, A.COMPLEX64) as rg:
+        offs = rg.offsets()
+        x = synth.uniform_complex(29, (len(rg),), np.complex64)
+        xt = torch.from_numpy(x).to("cuda:0")
+        s = torch.cuda.Stream()
+        for form in ("exec", "exec_graph"):
+            yt = torch.zeros_like(xt)
+            with torch.cuda.stream(s):
+                getattr(rg, form)(A.FORWARD, yt.data_ptr(), xt.data_ptr(), s.cuda_stream)
+            s.synchronize()
+            _check(sizes, offs, x, yt.cpu().numpy(), np.complex64)
