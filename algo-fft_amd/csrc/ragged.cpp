@@ -43,25 +43,27 @@ __global__ void k_rag_reset(int* counter) {
 }  // namespace afft
 
 // The launches form independent chains (the tasks of a job depend only on
-// earlier launches of the same chain): runtime-pass lengths, power-of-two
-// lengths, split Bluestein jobs, whole Bluestein jobs.  The first two chains
-// run their split jobs' launches, then their whole jobs' launches (whole
-// jobs depend on nothing, so they fill the chain's tail).  Whole Bluestein
-// jobs are a chain of their own: four launches of few blocks each, which
-// at the tail of the split Bluestein chain (the longest) added about 95 us
-// of mostly idle device to it.  Measured on the 4096-job mixed set: 0.561
-// ms with them on the Bluestein chain, 0.515-0.517 ms on their own chain
-// whatever its enqueue order and share; 0.604 ms with every whole-job launch
-// on a chain of its own.
+// earlier launches of the same chain): split runtime-pass lengths, power-of-
+// two lengths (split, then whole), split Bluestein jobs, and the whole jobs
+// of the runtime passes and of Bluestein.  Whole jobs depend on nothing, so
+// they can go on any chain; the fourth chain takes the ones that lengthened
+// the longest chains.  Measured on the 4096-job mixed set: 0.561 ms with the
+// whole Bluestein jobs at the tail of the split Bluestein chain (four
+// launches of few blocks each added about 95 us of mostly idle device to
+// it), 0.515-0.517 ms with them on a chain of their own, 0.478-0.481 ms
+// once the split kernels had sped up, and 0.468-0.470 ms with the runtime-
+// pass whole jobs moved there too from the tail of the split runtime-pass
+// chain (then the last to finish); 0.604 ms with every whole-job launch on
+// a chain of its own.
 constexpr int RAG_BRANCHES = 4;
 static int rag_branch(int kind) {
     switch (kind) {
     case RL_COLS:
-    case RL_ROWS:
-    case RL_WHOLE: return 0;
+    case RL_ROWS: return 0;
     case RL_COLS_P2:
     case RL_ROWS_P2:
     case RL_WHOLE_P2: return 1;
+    case RL_WHOLE:
     case RL_WHOLE_BLUE: return 3;
     default: return 2;   // RL_COLS_BLUE, RL_ROWS_BLUE, RL_COLS2
     }
@@ -446,13 +448,15 @@ static int ragged_build(afft_ragged_s& r) {
     AFFT_HIP(hipMalloc(&r.counter, RAG_COUNTERS * sizeof(int)));
     // Each persistent launch gets a share of the device's resident blocks,
     // so that the concurrent branches run side by side instead of the first
-    // one filling every CU until its queue drains.  Weights 4:5:7:2 (runtime
-    // passes, power-of-two cores, split and whole Bluestein jobs) over the branches
+    // one filling every CU until its queue drains.  Weights 4:5:7:3 (split
+    // runtime passes, power-of-two cores, split Bluestein, whole runtime-pass
+    // and Bluestein jobs) over the branches
     // present, 1.7 device-fulls in all, at most one device-full per branch;
     // a set with one branch keeps the whole device.  Shares swept on the
     // 4096-job mixed set: 0.561-0.61 ms over the settings tried with three
-    // chains; the whole Bluestein chain's weight (1 to 4) changed nothing.
-    static constexpr int weight[RAG_BRANCHES] = {4, 5, 7, 2};
+    // chains; with the current chains a fourth-chain weight of 2 to 4 gives
+    // 0.468-0.469 ms, 5 or 1.9 device-fulls 0.473 ms.
+    static constexpr int weight[RAG_BRANCHES] = {4, 5, 7, 3};
     bool present[RAG_BRANCHES] = {};
     for (const RagLaunch& l : r.launches) present[rag_branch(l.kind)] = true;
     int wsum = 0;
@@ -483,12 +487,12 @@ static int ragged_launch(afft_ragged_s& r, int dir, void* dst, const void* src, 
     AFFT_HIP(hipEventRecord(r.fork, s));
     for (int b = 1; b < RAG_BRANCHES; ++b)
         if (used[b]) AFFT_HIP(hipStreamWaitEvent(r.branch[b - 1], r.fork, 0));
-    // Enqueue the runtime-pass chain first (its generic kernels run at about
-    // half the per-point rate of the compile-time ones, so it finishes
-    // last), then split Bluestein, power-of-two, whole Bluestein; each branch
-    // keeps its own launch order.  Measured on the 4096-job mixed set:
-    // 0.508 ms against 0.515-0.518 ms with split Bluestein first; shares of
-    // 1.5 to 3.0 device-fulls 0.507-0.548 ms.
+    // Enqueue the split runtime-pass chain first (its generic kernels run at
+    // about half the per-point rate of the compile-time ones), then split
+    // Bluestein, power-of-two, whole jobs; each branch keeps its own launch
+    // order.  Measured on the 4096-job mixed set: 0.508 ms against
+    // 0.515-0.518 ms with split Bluestein first; shares of 1.5 to 3.0
+    // device-fulls 0.507-0.548 ms.
     static constexpr int order[RAG_BRANCHES] = {0, 2, 1, 3};
     for (int ob = 0; ob < RAG_BRANCHES; ++ob)
     for (size_t p = 0; p < r.launches.size(); ++p) {
