@@ -21,6 +21,32 @@ struct __attribute__((aligned(2 * sizeof(T)))) cx {
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
 
+// A read-only pointer into global memory.  A pointer that a kernel reads out
+// of a table in memory, rather than receives as an argument, is generic to
+// the compiler, which then reads through it with flat loads: those count
+// against the LDS operations as well, so every wait for one also drains the
+// LDS reads and writes in flight.  Reading through a gptr instead makes them
+// global loads.  It indexes and offsets like a const cx<U>*, and an element
+// read returns a value (copying the address-space-qualified object itself
+// is not allowed in host compilation).
+template <typename T>
+struct gptr;
+template <typename U>
+struct gptr<cx<U>> {
+    const __attribute__((address_space(1))) cx<U>* p;
+    __device__ __forceinline__ cx<U> operator[](long long i) const {
+        cx<U> r;
+        r.x = p[i].x;
+        r.y = p[i].y;
+        return r;
+    }
+    __device__ __forceinline__ gptr operator+(long long i) const { return gptr{p + i}; }
+};
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(const void* p) {
+    return gptr<T>{(const __attribute__((address_space(1))) T*)p};
+}
+
 // Non-temporal flags of a launch (KArgs::nt): streaming stores of the
 // output, and streaming loads of the input, for data too large to be re-read
 // from the last-level cache.

@@ -66,9 +66,8 @@ struct LdsCol {
     __device__ __forceinline__ int operator()(int n) const { return n * TPB + c; }
 };
 
-template <typename T, class S, int E, int DIR, int p, bool HOIST, class L>
-__device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
-                                          const L& li) {
+template <typename T, class S, int E, int DIR, int p, bool HOIST, class TW, class L>
+__device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, TW tw, const L& li) {
     constexpr int N = S::N;
     constexpr int TT = N / E;
     constexpr int R = S::rad[p];
@@ -84,9 +83,10 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
     constexpr int NL = NS == 1 ? 1 : (R == 16 ? 4 : R == 8 ? 3 : R - 1);
     cx<T> wl[B][NL];
     auto load_tw = [&]() {
+        if constexpr (NS > 1)
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            const cx<T>* __restrict__ w = tw + S::twoff(p) + (t + TT * b) % NS;
+            const TW w = tw + S::twoff(p) + (t + TT * b) % NS;
             if constexpr (R == 16 || R == 8) {
                 wl[b][0] = w[0];
                 wl[b][1] = w[NS];
@@ -146,18 +146,18 @@ __device__ __forceinline__ void core_pass(cx<T> (&v)[E], cx<T>* lds, int t, cons
     }
 }
 
-template <typename T, class S, int E, int DIR, bool HOIST, class L, int... Ps>
-__device__ __forceinline__ void core_all(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
-                                         const L& li, std::integer_sequence<int, Ps...>) {
+template <typename T, class S, int E, int DIR, bool HOIST, class TW, class L, int... Ps>
+__device__ __forceinline__ void core_all(cx<T> (&v)[E], cx<T>* lds, int t, TW tw, const L& li,
+                                         std::integer_sequence<int, Ps...>) {
     (core_pass<T, S, E, DIR, Ps, HOIST>(v, lds, t, tw, li), ...);
 }
 
 // Full transform.  PRESYNC adds a barrier before the first LDS write, needed
 // when the LDS slab was read earlier in the same kernel.  HOIST issues each
-// pass's twiddle loads before its barrier (see core_pass).
-template <typename T, class S, int E, int DIR, bool PRESYNC = false, bool HOIST = false, class L>
-__device__ __forceinline__ void core(cx<T> (&v)[E], cx<T>* lds, int t, const cx<T>* __restrict__ tw,
-                                     const L& li) {
+// pass's twiddle loads before its barrier (see core_pass).  tw is the
+// twiddle table: a const cx<T>* or a gptr<cx<T>>.
+template <typename T, class S, int E, int DIR, bool PRESYNC = false, bool HOIST = false, class TW, class L>
+__device__ __forceinline__ void core(cx<T> (&v)[E], cx<T>* lds, int t, TW tw, const L& li) {
     if constexpr (PRESYNC && S::P > 1) __syncthreads();
     core_all<T, S, E, DIR, HOIST>(v, lds, t, tw, li, std::make_integer_sequence<int, S::P>{});
 }

@@ -28,6 +28,17 @@ const void* gen_ip_kernel(int prec, int dir) {
     return dir < 0 ? (const void*)&k_gen_ip<double, -1> : (const void*)&k_gen_ip<double, 1>;
 }
 
+// VAR_4A or VAR_4B only.
+const void* gen_ipg_kernel(int prec, int var, int dir) {
+    if (var != VAR_4A && var != VAR_4B) return nullptr;
+    if (prec == 0) {
+        if (var == VAR_4A) return dir < 0 ? (const void*)&k_gen_ipg<float, VAR_4A, -1> : (const void*)&k_gen_ipg<float, VAR_4A, 1>;
+        return dir < 0 ? (const void*)&k_gen_ipg<float, VAR_4B, -1> : (const void*)&k_gen_ipg<float, VAR_4B, 1>;
+    }
+    if (var == VAR_4A) return dir < 0 ? (const void*)&k_gen_ipg<double, VAR_4A, -1> : (const void*)&k_gen_ipg<double, VAR_4A, 1>;
+    return dir < 0 ? (const void*)&k_gen_ipg<double, VAR_4B, -1> : (const void*)&k_gen_ipg<double, VAR_4B, 1>;
+}
+
 const void* copy_kernel(int prec) {
     return prec == 0 ? (const void*)&k_copy<float> : (const void*)&k_copy<double>;
 }

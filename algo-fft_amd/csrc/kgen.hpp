@@ -55,8 +55,8 @@ __device__ __forceinline__ void dft_rt(int R, cx<T>* v) {
 // apply it: the products' registers pushed the in-place rows from 23 to
 // 39-43 spilled VGPRs and the two-buffer columns from 3 to 2 waves per SIMD
 // (N=6000 0.365 -> 0.436 ms, the ragged set 0.499 -> 0.562 ms).
-template <int R, typename T>
-__device__ __forceinline__ void tw_powers(const cx<T>* __restrict__ w, cx<T> (&ww)[R]) {
+template <int R, typename T, class W>
+__device__ __forceinline__ void tw_powers(W w, cx<T> (&ww)[R]) {
     if constexpr (R <= 3) {
 #pragma unroll
         for (int r = 1; r < R; ++r) ww[r] = w[r - 1];
@@ -87,8 +87,8 @@ __device__ __forceinline__ void tw_powers(const cx<T>* __restrict__ w, cx<T> (&w
 }
 
 // v[r] *= w^r (r = 1 .. R-1) for the butterfly whose table entries start at w.
-template <int R, int DIR, typename T>
-__device__ __forceinline__ void tw_apply(const cx<T>* __restrict__ w, cx<T>* v) {
+template <int R, int DIR, typename T, class W>
+__device__ __forceinline__ void tw_apply(W w, cx<T>* v) {
     if constexpr (sizeof(T) == 4) {
 #pragma unroll
         for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], w[r - 1]);
@@ -105,9 +105,8 @@ __device__ __forceinline__ void tw_apply(const cx<T>* __restrict__ w, cx<T>* v) 
 // the N/R in the pass; load(n) gives input point n, store(k, v, r) takes
 // output point k = (j/Ns)*Ns*R + (j mod Ns) + r*Ns (r counts 0 .. R-1 within
 // the butterfly).
-template <int R, int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, unsigned NSM, const cx<T>* __restrict__ tw,
-                                        Load load, Store store) {
+template <int R, int DIR, typename T, class W, class Load, class Store>
+__device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, unsigned NSM, W tw, Load load, Store store) {
     const int NR = N / R;
     for (int j = t; j < NR; j += TT) {
         cx<T> v[R];
@@ -122,17 +121,17 @@ __device__ __forceinline__ void rt_pass(int t, int TT, int N, int NS, unsigned N
     }
 }
 
-template <int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS, unsigned NSM, const cx<T>* tw,
-                                            Load load, Store store) {
+template <int DIR, typename T, class W, class Load, class Store>
+__device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS, unsigned NSM, W tw, Load load,
+                                            Store store) {
     switch (R) {
-    case 2: rt_pass<2, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
-    case 3: rt_pass<3, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
-    case 4: rt_pass<4, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
-    case 5: rt_pass<5, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
-    case 7: rt_pass<7, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
-    case 8: rt_pass<8, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
-    case 16: rt_pass<16, DIR>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 2: rt_pass<2, DIR, T>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 3: rt_pass<3, DIR, T>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 4: rt_pass<4, DIR, T>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 5: rt_pass<5, DIR, T>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 7: rt_pass<7, DIR, T>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 8: rt_pass<8, DIR, T>(t, TT, N, NS, NSM, tw, load, store); break;
+    case 16: rt_pass<16, DIR, T>(t, TT, N, NS, NSM, tw, load, store); break;
     default: break;
     }
 }
@@ -142,9 +141,9 @@ __device__ __forceinline__ void rt_pass_any(int R, int t, int TT, int N, int NS,
 // applies twiddles and the codelet, and after a barrier of the whole block
 // writes output point (j/Ns)*Ns*R + j%Ns + r*Ns (store(k, v)).  Every thread
 // of the block calls this; `active` threads do the work.
-template <int R, int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, int NS, unsigned NSM,
-                                           const cx<T>* __restrict__ tw, Load load, Store store) {
+template <int R, int DIR, typename T, class W, class Load, class Store>
+__device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, int NS, unsigned NSM, W tw, Load load,
+                                           Store store) {
     constexpr int MB = (16 + R - 1) / R;
     const int NR = N / R;
     cx<T> v[MB][R];
@@ -176,17 +175,17 @@ __device__ __forceinline__ void rt_pass_ip(bool active, int t, int TT, int N, in
     }
 }
 
-template <int DIR, typename T, class Load, class Store>
-__device__ __forceinline__ void rt_pass_ip_any(int R, bool active, int t, int TT, int N, int NS, unsigned NSM,
-                                               const cx<T>* tw, Load load, Store store) {
+template <int DIR, typename T, class W, class Load, class Store>
+__device__ __forceinline__ void rt_pass_ip_any(int R, bool active, int t, int TT, int N, int NS, unsigned NSM, W tw,
+                                               Load load, Store store) {
     switch (R) {
-    case 2: rt_pass_ip<2, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 3: rt_pass_ip<3, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 4: rt_pass_ip<4, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 5: rt_pass_ip<5, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 7: rt_pass_ip<7, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 8: rt_pass_ip<8, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
-    case 16: rt_pass_ip<16, DIR>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 2: rt_pass_ip<2, DIR, T>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 3: rt_pass_ip<3, DIR, T>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 4: rt_pass_ip<4, DIR, T>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 5: rt_pass_ip<5, DIR, T>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 7: rt_pass_ip<7, DIR, T>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 8: rt_pass_ip<8, DIR, T>(active, t, TT, N, NS, NSM, tw, load, store); break;
+    case 16: rt_pass_ip<16, DIR, T>(active, t, TT, N, NS, NSM, tw, load, store); break;
     default: {   // radix 1: the copy of a one-point transform
         cx<T> v0 = mk<T>(0, 0);
         if (active && t < N) v0 = load(t);
@@ -258,11 +257,11 @@ __global__ void __launch_bounds__((VAR == VAR_ROW && sizeof(T) == 4 ? GEN_ROWTHR
             }
         };
         if (p == 0) {
-            if (last) rt_pass_any<DIR>(R, t, TPT, N, NS, NSM, twp, load_g, store_o);
-            else rt_pass_any<DIR>(R, t, TPT, N, NS, NSM, twp, load_g, store_s);
+            if (last) rt_pass_any<DIR, T>(R, t, TPT, N, NS, NSM, twp, load_g, store_o);
+            else rt_pass_any<DIR, T>(R, t, TPT, N, NS, NSM, twp, load_g, store_s);
         } else {
-            if (last) rt_pass_any<DIR>(R, t, TPT, N, NS, NSM, twp, load_s, store_o);
-            else rt_pass_any<DIR>(R, t, TPT, N, NS, NSM, twp, load_s, store_s);
+            if (last) rt_pass_any<DIR, T>(R, t, TPT, N, NS, NSM, twp, load_s, store_o);
+            else rt_pass_any<DIR, T>(R, t, TPT, N, NS, NSM, twp, load_s, store_s);
         }
         __syncthreads();
     }
@@ -331,10 +330,104 @@ __global__ void __launch_bounds__(GenIP<T>::THREADS) k_gen_ip(GArgs g) {
             st_out(dst + k, y, a.nt);
         };
         if (p == g.npass - 1) {
-            rt_pass_ip_any<DIR>(R, true, t, TT, N, NS, NSM, twp, load_s, store_o);
+            rt_pass_ip_any<DIR, T>(R, true, t, TT, N, NS, NSM, twp, load_s, store_o);
         } else {
-            rt_pass_ip_any<DIR>(R, true, t, TT, N, NS, NSM, twp, load_s, store_s);
+            rt_pass_ip_any<DIR, T>(R, true, t, TT, N, NS, NSM, twp, load_s, store_s);
             __syncthreads();
+        }
+    }
+}
+
+// The two four-step passes (VAR_4A columns, VAR_4B rows with the
+// transposing store) with G = blockDim.x / TPT transforms per block and
+// every pass in place in ONE LDS buffer of G*N points, for N <= 16*TPT: each
+// thread holds all its butterflies of a pass in registers across the pass's
+// barrier (rt_pass_ip).  Half the LDS of k_gen lets twice the blocks share
+// a CU: k_gen's four-step passes hold 2-3 blocks of 4 waves per CU, latency
+// bound with the vector ALUs under half busy.  The input is staged into LDS
+// first with up to 16 loads of a thread in flight.
+template <typename T>
+struct GenIPG {
+    static constexpr int THREADS = GEN_THREADS;
+    // waves per SIMD the register budget allows (128 VGPRs for complex64,
+    // 256 for complex128)
+    static constexpr int WAVES_PER_SIMD = sizeof(T) == 4 ? 4 : 2;
+};
+
+template <typename T, int VAR, int DIR>
+__global__ void __launch_bounds__(GenIPG<T>::THREADS, GenIPG<T>::WAVES_PER_SIMD) k_gen_ipg(GArgs g) {
+    extern __shared__ unsigned char smraw[];
+    cx<T>* sm = (cx<T>*)smraw;
+    const KArgs a = g.k;
+    const int N = g.n;
+    const int TPT = g.tpt;
+    const int lgt = __ffs(TPT) - 1;
+    const int lgg = __ffs(blockDim.x) - 1 - lgt;   // blockDim.x is a power of two
+    const int G = 1 << lgg;
+    static_assert(VAR == VAR_4A || VAR == VAR_4B, "four-step passes only");
+    const int tid = threadIdx.x;
+    // Both layouts hold transform c at sm[c*N ..] (column-interleaved slots
+    // made the column pass's addressing spill).  The passes run TPT
+    // consecutive threads per transform; loads and stores of strided columns
+    // (VAR_4A input, both outputs) run lanes along c, so that neighbouring
+    // lanes touch neighbouring addresses.
+    const int c = tid >> lgt, t = tid & (TPT - 1);
+    const int c2 = tid & (G - 1), t2 = tid >> lgg;
+    const long long b0 = (long long)blockIdx.x * G;
+    const cx<T>* __restrict__ in = (const cx<T>*)a.in + (long long)blockIdx.y * a.in_bd;
+    cx<T>* __restrict__ out = (cx<T>*)a.out + (long long)blockIdx.y * a.out_bd;
+    const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+    constexpr bool ROWIN = VAR == VAR_4B;
+    {
+        const int sc = ROWIN ? c : c2, st = ROWIN ? t : t2;
+        const bool act = b0 + sc < a.count;
+        const bool ntl = a.nt & NT_LOAD;
+        const cx<T>* q = ROWIN ? in + (b0 + sc) * a.in_cd + st : in + b0 + sc + (long long)st * a.in_nd;
+        const long long step = ROWIN ? (long long)TPT : (long long)TPT * a.in_nd;
+        cx<T> v[16];
+#pragma unroll
+        for (int h = 0; h < 16; ++h, q += step) {
+            const int n = st + h * TPT;
+            v[h] = (act && n < N) ? (ntl ? ld_nt(q) : *q) : mk<T>(0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < 16; ++h)
+            if (st + h * TPT < N) sm[sc * N + st + h * TPT] = v[h];
+    }
+    __syncthreads();
+    const T scale = (T)a.scale;
+    cx<T>* row = sm + c * N;
+    for (int p = 0; p < g.npass; ++p) {
+        const int R = g.rad[p];
+        const int NS = g.ns[p];
+        const unsigned NSM = g.nsm[p];
+        const cx<T>* twp = tw + g.twoff[p];
+        auto load_s = [&](int n) -> cx<T> { return row[n]; };
+        auto store_s = [&](int k, cx<T> y) { row[k] = y; };
+        rt_pass_ip_any<DIR, T>(R, true, t, TPT, N, NS, NSM, twp, load_s, store_s);
+        if (p < g.npass - 1) __syncthreads();
+    }
+    {
+        // column stores: one pointer add per point.  VAR_4A applies
+        // W_L^{k*col} here: a table value every fourth point of a thread, the
+        // step W_L^{TPT*col} in between (in the last pass it spilled).
+        __syncthreads();
+        if (b0 + c2 < a.count) {
+            cx<T>* o = out + b0 + c2 + (long long)t2 * a.out_nd;
+            const long long step = (long long)TPT * a.out_nd;
+            const cx<T>* r2 = sm + c2 * N;
+            const long long col = b0 + c2 + a.tw4_c0;
+            cx<T> w = mk<T>(1, 0), wst = mk<T>(1, 0);
+            if constexpr (VAR == VAR_4A) wst = tw4<T>(a, (long long)TPT * col);
+            for (int k = t2, i = 0; k < N; k += TPT, ++i, o += step) {
+                cx<T> y = r2[k];
+                if constexpr (VAR == VAR_4A) {
+                    if ((i & 3) == 0) w = tw4<T>(a, (long long)k * col);
+                    else w = cmul(w, wst);
+                    y = twmul<DIR>(y, w);
+                }
+                st_out(o, scl(y, scale), a.nt);
+            }
         }
     }
 }

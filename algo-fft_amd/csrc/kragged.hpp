@@ -126,8 +126,8 @@ struct RagArgs {
 
 template <typename T>
 __device__ __forceinline__ cx<T> rag_tw(const RJob& jb, long long i) {
-    const cx<T>* lo = (const cx<T>*)jb.twlo;
-    const cx<T>* hi = (const cx<T>*)jb.twhi;
+    const gptr<cx<T>> lo = as_global<cx<T>>(jb.twlo);
+    const gptr<cx<T>> hi = as_global<cx<T>>(jb.twhi);
     return cmul(lo[i & ((1LL << jb.shift) - 1)], hi[i >> jb.shift]);
 }
 
@@ -142,16 +142,16 @@ __device__ __forceinline__ void rag_passes(const RSched& sc, int L, int P, int c
     const int grp = tid >> (__ffs(TPT) - 1), t = tid & (TPT - 1);
     const bool active = grp < cnt;
     cx<T>* row = buf + grp * P;
-    const cx<T>* __restrict__ tw = (const cx<T>*)sc.tw;
+    const gptr<cx<T>> tw = as_global<cx<T>>(sc.tw);
     for (int p = 0; p < sc.npass; ++p) {
         auto load = [&](int n) -> cx<T> { return row[n]; };
         auto store = [&](int k, cx<T> v) { row[k] = v; };
         auto fin = [&](int k, cx<T> v) { sink(grp, k, v); };
         const int R = sc.rad[p];
         if (p == sc.npass - 1)
-            rt_pass_ip_any<DIR>(R, active, t, TPT, L, sc.ns[p], sc.nsm[p], tw + sc.twoff[p], load, fin);
+            rt_pass_ip_any<DIR, T>(R, active, t, TPT, L, sc.ns[p], sc.nsm[p], tw + sc.twoff[p], load, fin);
         else
-            rt_pass_ip_any<DIR>(R, active, t, TPT, L, sc.ns[p], sc.nsm[p], tw + sc.twoff[p], load, store);
+            rt_pass_ip_any<DIR, T>(R, active, t, TPT, L, sc.ns[p], sc.nsm[p], tw + sc.twoff[p], load, store);
         __syncthreads();
     }
 }
@@ -285,8 +285,8 @@ __device__ void rag_whole_blue(const RagArgs a, const RJob& jb, cx<T>* lds) {
     cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
     const int m = jb.m, n = jb.n, tid = threadIdx.x;
     const bool inv = a.dir > 0;
-    const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
-    const cx<T>* __restrict__ filt = (const cx<T>*)jb.filt[inv ? 1 : 0];
+    const gptr<cx<T>> chirp = as_global<cx<T>>(jb.chirp);
+    const gptr<cx<T>> filt = as_global<cx<T>>(jb.filt[inv ? 1 : 0]);
     rag_staged<sizeof(cx2<T>)>(
         flat_steps(m),
         [&](int s) {
@@ -453,7 +453,7 @@ __device__ __forceinline__ void p2_cols(const RagArgs a, const RJob& jb, const R
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
     cx<T> v[E];
     if constexpr (BLUE) {
-        const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
+        const gptr<cx<T>> chirp = as_global<cx<T>>(jb.chirp);
         const bool inv = a.dir > 0;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -465,7 +465,7 @@ __device__ __forceinline__ void p2_cols(const RagArgs a, const RJob& jb, const R
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = x[m2 * (t + TT * e) + col];
     }
-    core<T, S, E, SD>(v, lds, t, (const cx<T>*)jb.s[0].tw, LdsCol<G>(c));
+    core<T, S, E, SD>(v, lds, t, as_global<cx<T>>(jb.s[0].tw), LdsCol<G>(c));
     cx<T>* __restrict__ sp = (cx<T>*)a.scratch + jb.scr_off + col;
     // W_m^{k1*col} for k1 = t + TT*e: a table value every fourth e, the step
     // W_m^{TT*col} in between (lanes differ in col: scattered table lines)
@@ -500,7 +500,7 @@ __device__ __forceinline__ void p2_rows(const RagArgs a, const RJob& jb, const R
     cx<T> v[E];
     p2_row_load<T, L>((const cx<T>*)a.scratch + jb.scr_off + (long long)(r0 + c) * L, t, v);
     const LdsRow<L, PADSH> li(c);
-    core<T, S, E, SD>(v, lds, t, (const cx<T>*)jb.s[1].tw, li);
+    core<T, S, E, SD>(v, lds, t, as_global<cx<T>>(jb.s[1].tw), li);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) lds[li(t + TT * e)] = v[e];
@@ -524,8 +524,8 @@ __device__ __forceinline__ void p2_rows_blue(const RagArgs a, const RJob& jb, co
     const int tid = threadIdx.x, c = tid / TT, t = tid % TT;
     const int k1 = tk.first + c;
     cx<T>* __restrict__ row = (cx<T>*)a.scratch + jb.scr_off + (long long)k1 * L;
-    const cx<T>* __restrict__ frow = (const cx<T>*)jb.filt[a.dir > 0 ? 1 : 0] + (long long)k1 * L;
-    const cx<T>* __restrict__ tw = (const cx<T>*)jb.s[1].tw;
+    const gptr<cx<T>> frow = as_global<cx<T>>(jb.filt[a.dir > 0 ? 1 : 0]) + (long long)k1 * L;
+    const gptr<cx<T>> tw = as_global<cx<T>>(jb.s[1].tw);
     cx<T> v[E];
     p2_row_load<T, L>(row, t, v);
     const LdsRow<L, PADSH> li(c);
@@ -568,7 +568,7 @@ __device__ __forceinline__ void p2_whole(const RagArgs a, const RTask& tk, cx<T>
     const bool act = c < tk.count;
     const int jid = a.joblist[tk.first + (act ? c : 0)];
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + a.jobs[jid].in_off;
-    const cx<T>* __restrict__ tw = (const cx<T>*)a.jobs[a.joblist[tk.first]].s[0].tw;
+    const gptr<cx<T>> tw = as_global<cx<T>>(a.jobs[a.joblist[tk.first]].s[0].tw);
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = x[t + TT * e];
@@ -605,9 +605,9 @@ __device__ __forceinline__ void p2_whole_blue(const RagArgs a, const RTask& tk, 
     const bool inv = a.dir > 0;
     const int n = jb.n;
     const cx<T>* __restrict__ x = (const cx<T>*)a.in + jb.in_off;
-    const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
-    const cx<T>* __restrict__ filt = (const cx<T>*)jb.filt[inv ? 1 : 0];
-    const cx<T>* __restrict__ tw = (const cx<T>*)a.jobs[a.joblist[tk.first]].s[0].tw;
+    const gptr<cx<T>> chirp = as_global<cx<T>>(jb.chirp);
+    const gptr<cx<T>> filt = as_global<cx<T>>(jb.filt[inv ? 1 : 0]);
+    const gptr<cx<T>> tw = as_global<cx<T>>(a.jobs[a.joblist[tk.first]].s[0].tw);
     const LdsRow<L, P2Whole<T, L>::PADSH> li(c);
     cx<T> v[E];
 #pragma unroll
@@ -644,10 +644,10 @@ __device__ __forceinline__ void p2_cols2(const RagArgs a, const RJob& jb, const 
     cx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = sp[m2 * (t + TT * e)];
-    core<T, S, E, 1>(v, lds, t, (const cx<T>*)jb.s[0].tw, LdsCol<G>(c));
+    core<T, S, E, 1>(v, lds, t, as_global<cx<T>>(jb.s[0].tw), LdsCol<G>(c));
     const bool inv = a.dir > 0;
     const T sc = inv ? T(1) / T(jb.n) : T(1);
-    const cx<T>* __restrict__ chirp = (const cx<T>*)jb.chirp;
+    const gptr<cx<T>> chirp = as_global<cx<T>>(jb.chirp);
     cx<T>* __restrict__ y = (cx<T>*)a.out + jb.out_off;
 #pragma unroll
     for (int e = 0; e < E; ++e) {

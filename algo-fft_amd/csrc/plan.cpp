@@ -434,6 +434,7 @@ static int emit_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
 // ---- runtime-size mixed radix ------------------------------------------------
 const void* gen_kernel(int prec, int var, int dir);
 const void* gen_ip_kernel(int prec, int dir);
+const void* gen_ipg_kernel(int prec, int var, int dir);
 const void* copy_kernel(int prec);
 
 // Radix schedule for a 2/3/5/7-smooth n: radix-16 passes and the
@@ -507,6 +508,7 @@ int64_t next_pow2(int64_t n) {
 struct GenCfg {
     int tpt, g, threads;
     int lds;
+    bool ipg = false;   // k_gen_ipg: G transforms per block, passes in place in one buffer
 };
 
 // Two LDS buffers of G transforms, TPT threads each.  Row layouts: about 8
@@ -558,7 +560,16 @@ static bool gen_single_fits(int prec, int64_t n, bool rows) {
     return n <= gen_cap(prec) || (rows && n <= gen_row_cap(prec));
 }
 
-static GenCfg gen_cfg(int64_t n, int prec, bool rows) {
+// With ipg, both four-step passes run in place in one LDS buffer
+// (k_gen_ipg) when they hold at most 16 points per thread: half the LDS of
+// the two-buffer kernel, so twice the blocks per CU.  Measured on 512 MiB
+// batches: complex64 N=100000 (250 x 400) 0.990 -> 0.831 ms, N=250000 (500
+// x 500) 1.012 -> 0.829 ms, complex128 N=100000 0.952 -> 0.700 ms.  The
+// strided axes of N-D plans stay on the two-buffer kernel: their long
+// columns leave 1-2 columns per block, and the in-place form's separate
+// store phase lost 2-7 % there (complex64 1200 x 1500 1.322 -> 1.345 ms,
+// complex128 0.752 -> 0.803 ms).
+static GenCfg gen_cfg(int64_t n, int prec, bool rows, bool ipg = false) {
     if (rows && gen_use_ip(n, prec)) {
         GenCfg c;
         c.tpt = c.threads = gen_ip_threads(n, prec);
@@ -566,10 +577,18 @@ static GenCfg gen_cfg(int64_t n, int prec, bool rows) {
         c.lds = (int)(n * (prec ? 16 : 8));
         return c;
     }
-    return gen_cfg_2b(n, prec, rows);
+    GenCfg c = gen_cfg_2b(n, prec, rows);
+    if (ipg && !rows && n <= 16 * (int64_t)c.tpt) {
+        c.ipg = true;
+        c.lds = (int)(c.g * n * (prec ? 16 : 8));
+    }
+    return c;
 }
 
-static int gen_args(Plan& p, int64_t n, bool rows, GArgs& g, int* tpt, int* lds, int* threads) {
+// ipg: may choose the in-place grouped kernel (four-step passes); *ipg says
+// whether it did, and lds is then that kernel's.
+static int gen_args(Plan& p, int64_t n, bool rows, GArgs& g, int* tpt, int* lds, int* threads,
+                    bool* ipg = nullptr) {
     std::vector<int> rad = smooth_schedule(n);
     if (rad.empty() || (int)rad.size() > GEN_MAXPASS) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "no radix schedule");
     KernelInfo fake{};
@@ -607,12 +626,13 @@ static int gen_args(Plan& p, int64_t n, bool rows, GArgs& g, int* tpt, int* lds,
         rc = upload_table(p, tw.data(), tw.size() * 8, &g.k.tw);
     }
     if (rc) return rc;
-    GenCfg c = gen_cfg(n, p.desc.precision, rows);
+    GenCfg c = gen_cfg(n, p.desc.precision, rows, ipg != nullptr);
     g.n = (int)n;
     g.tpt = c.tpt;
     *tpt = c.tpt;
     *lds = c.lds;
     *threads = c.threads;
+    if (ipg) *ipg = c.ipg;
     return AFFT_OK;
 }
 
@@ -723,19 +743,20 @@ static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& alg
     const KernelInfo* kb = (is_pow2(n2) && n2 >= 2) ? find_kernel(prec, (int)n2, VAR_4B, dir) : nullptr;
     GArgs ga{}, gb{};
     int tpa = 1, ldsa = 0, tha = 1, tpb = 1, ldsb = 0, thb = 1;
+    bool ipa = false, ipb = false;
     if (ka) {
         if ((rc = schedule_twiddles(p, ka, &ga.k.tw))) return rc;
-    } else if ((rc = gen_args(p, n1, false, ga, &tpa, &ldsa, &tha))) {
+    } else if ((rc = gen_args(p, n1, false, ga, &tpa, &ldsa, &tha, &ipa))) {
         return rc;
     }
     if (kb) {
         if ((rc = schedule_twiddles(p, kb, &gb.k.tw))) return rc;
-    } else if ((rc = gen_args(p, n2, false, gb, &tpb, &ldsb, &thb))) {
+    } else if ((rc = gen_args(p, n2, false, gb, &tpb, &ldsb, &thb, &ipb))) {
         return rc;
     }
     if ((rc = split_twiddles(p, prec, j.n, &ga.k.tw4_shift, &ga.k.tw4_lo, &ga.k.tw4_hi))) return rc;
-    const void* fa = ka ? ka->fn : gen_kernel(prec, VAR_4A, dir);
-    const void* fb = kb ? kb->fn : gen_kernel(prec, VAR_4B, dir);
+    const void* fa = ka ? ka->fn : ipa ? gen_ipg_kernel(prec, VAR_4A, dir) : gen_kernel(prec, VAR_4A, dir);
+    const void* fb = kb ? kb->fn : ipb ? gen_ipg_kernel(prec, VAR_4B, dir) : gen_kernel(prec, VAR_4B, dir);
     if ((!ka && (rc = prep_dynamic_lds(fa, ldsa))) || (!kb && (rc = prep_dynamic_lds(fb, ldsb)))) return rc;
     ga.k.tw4_L = j.n;
     ga.k.count = n2;

@@ -149,3 +149,35 @@ def test_in_place_rows_in_nd_plans(ctx, dims):
         p.forward(out, x)
         algo = p.describe()
     assert rel_l2(out, O.fftn(x.astype(np.complex128))) <= c64_tol(int(np.prod(dims))), algo
+
+
+# Smooth four-step plans, whose runtime-radix passes run in place in one LDS
+# buffer holding several columns (or rows) per block: column counts that are
+# not a multiple of the columns per block, one point per thread (n1 = 3), a
+# compile-time column pass beside a runtime row pass (n1 = 8), and a row
+# pass above 16 points per thread (complex64 n2 = 6000, on the two-buffer
+# kernel; complex128 factors stop at 4096 points).
+FOURSTEP_SPLITS = [(3000, 3), (3000, 250), (1000, 8), (2187, 27), (5000, 2), (100000, 0), (250000, 0)]
+FOURSTEP_CASES = ([(A.COMPLEX64, n, s) for n, s in FOURSTEP_SPLITS + [(12000, 2)]]
+                  + [(A.COMPLEX128, n, s) for n, s in FOURSTEP_SPLITS])
+
+
+@pytest.mark.parametrize("prec,n,split", FOURSTEP_CASES)
+def test_mixed_fourstep_passes(ctx, prec, n, split):
+    dt = np.complex64 if prec == A.COMPLEX64 else np.complex128
+    batch = max(1, min(5, 300000 // n))
+    x = synth.uniform_complex(11 * n + split, (batch, n), dt)
+    with A.Plan.from_desc(ctx, (n,), precision=prec, batch=batch, strategy=A.STRATEGY_SIXSTEP, split=split) as p:
+        algo = p.describe()
+        got = np.empty_like(x)
+        p.forward(got, x)
+        back = np.empty_like(x)
+        p.inverse(back, got)
+    assert "mixed4step" in algo, algo
+    if split:
+        assert f"mixed4step{split}x{n // split}" in algo, algo
+    tol = c64_tol(n) if prec == A.COMPLEX64 else C128_TOL
+    err = rel_l2(got, O.fft_batch(x.astype(np.complex128)))
+    print(f"n={n} split={split} algo={algo} relL2={err:.2e}")
+    assert err <= tol, (n, split, err, algo)
+    assert rel_l2(back, O.fft_batch(got.astype(np.complex128), inverse=True)) <= tol, (n, split, algo)
