@@ -348,7 +348,9 @@ __global__ void __launch_bounds__(GenIP<T>::THREADS) k_gen_ip(GArgs g) {
 // first with up to 16 loads of a thread in flight.
 template <typename T>
 struct GenIPG {
-    static constexpr int THREADS = GEN_THREADS;
+    // largest block: blocks grow to 512 threads where 256 would hold fewer
+    // than 16 transforms, for longer column segments
+    static constexpr int THREADS = 2 * GEN_THREADS;
     // waves per SIMD the register budget allows (128 VGPRs for complex64,
     // 256 for complex128)
     static constexpr int WAVES_PER_SIMD = sizeof(T) == 4 ? 4 : 2;

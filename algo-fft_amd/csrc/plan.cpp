@@ -580,6 +580,17 @@ static GenCfg gen_cfg(int64_t n, int prec, bool rows, bool ipg = false) {
     GenCfg c = gen_cfg_2b(n, prec, rows);
     if (ipg && !rows && n <= 16 * (int64_t)c.tpt) {
         c.ipg = true;
+        // Blocks of 512 threads where 256 would hold fewer than 16
+        // (complex64) / 4 (complex128) transforms: longer column segments.
+        // Measured on 512 MiB batches (256 -> 512 threads), complex64:
+        // N=36000 (32 x 1125) 1.45 -> 0.85 ms, N=10^6 (1000 x 1000) 1.22 ->
+        // 0.72 ms, N=100000 0.82 -> 0.71 ms, but N=24000 (64 x 375) 0.60 ->
+        // 0.62 ms and N=162000 (400 x 405) 0.83 -> 0.88 ms; complex128
+        // N=36000 0.84 -> 0.69 ms, while N=10^6 (4 transforms a block at 256)
+        // went 0.71 -> 0.78 ms and N=100000 0.70 -> 0.77 ms.
+        if (c.g < (prec ? 4 : 16))
+            c.threads = std::min(2 * c.threads, prec ? GenIPG<double>::THREADS : GenIPG<float>::THREADS);
+        c.g = c.threads / c.tpt;
         c.lds = (int)(c.g * n * (prec ? 16 : 8));
     }
     return c;

@@ -157,7 +157,7 @@ def test_in_place_rows_in_nd_plans(ctx, dims):
 # compile-time column pass beside a runtime row pass (n1 = 8), and a row
 # pass above 16 points per thread (complex64 n2 = 6000, on the two-buffer
 # kernel; complex128 factors stop at 4096 points).
-FOURSTEP_SPLITS = [(3000, 3), (3000, 250), (1000, 8), (2187, 27), (5000, 2), (100000, 0), (250000, 0)]
+FOURSTEP_SPLITS = [(3000, 3), (3000, 250), (1000, 8), (2187, 27), (5000, 2), (36000, 0), (100000, 0), (250000, 0)]
 FOURSTEP_CASES = ([(A.COMPLEX64, n, s) for n, s in FOURSTEP_SPLITS + [(12000, 2)]]
                   + [(A.COMPLEX128, n, s) for n, s in FOURSTEP_SPLITS])
 
