@@ -28,8 +28,12 @@ const void* gen_ip_kernel(int prec, int dir) {
     return dir < 0 ? (const void*)&k_gen_ip<double, -1> : (const void*)&k_gen_ip<double, 1>;
 }
 
-// VAR_4A or VAR_4B only.
+// VAR_COL, VAR_4A or VAR_4B only.
 const void* gen_ipg_kernel(int prec, int var, int dir) {
+    if (var == VAR_COL) {
+        if (prec == 0) return dir < 0 ? (const void*)&k_gen_ipg<float, VAR_COL, -1> : (const void*)&k_gen_ipg<float, VAR_COL, 1>;
+        return dir < 0 ? (const void*)&k_gen_ipg<double, VAR_COL, -1> : (const void*)&k_gen_ipg<double, VAR_COL, 1>;
+    }
     if (var != VAR_4A && var != VAR_4B) return nullptr;
     if (prec == 0) {
         if (var == VAR_4A) return dir < 0 ? (const void*)&k_gen_ipg<float, VAR_4A, -1> : (const void*)&k_gen_ipg<float, VAR_4A, 1>;

@@ -338,12 +338,13 @@ __global__ void __launch_bounds__(GenIP<T>::THREADS) k_gen_ip(GArgs g) {
     }
 }
 
-// The two four-step passes (VAR_4A columns, VAR_4B rows with the
-// transposing store) with G = blockDim.x / TPT transforms per block and
+// Column layouts (VAR_COL strided axes, the four-step passes VAR_4A with
+// the four-step twiddle and VAR_4B rows with the transposing store) with
+// G = blockDim.x / TPT transforms per block and
 // every pass in place in ONE LDS buffer of G*N points, for N <= 16*TPT: each
 // thread holds all its butterflies of a pass in registers across the pass's
 // barrier (rt_pass_ip).  Half the LDS of k_gen lets twice the blocks share
-// a CU: k_gen's four-step passes hold 2-3 blocks of 4 waves per CU, latency
+// a CU: k_gen's column passes hold 2-3 blocks of 4 waves per CU, latency
 // bound with the vector ALUs under half busy.  The input is staged into LDS
 // first with up to 16 loads of a thread in flight.
 template <typename T>
@@ -366,13 +367,13 @@ __global__ void __launch_bounds__(GenIPG<T>::THREADS, GenIPG<T>::WAVES_PER_SIMD)
     const int lgt = __ffs(TPT) - 1;
     const int lgg = __ffs(blockDim.x) - 1 - lgt;   // blockDim.x is a power of two
     const int G = 1 << lgg;
-    static_assert(VAR == VAR_4A || VAR == VAR_4B, "four-step passes only");
+    static_assert(VAR == VAR_4A || VAR == VAR_4B || VAR == VAR_COL, "column layouts only");
     const int tid = threadIdx.x;
-    // Both layouts hold transform c at sm[c*N ..] (column-interleaved slots
-    // made the column pass's addressing spill).  The passes run TPT
+    // Every layout holds transform c at sm[c*N ..] (column-interleaved
+    // slots made the column pass's addressing spill).  The passes run TPT
     // consecutive threads per transform; loads and stores of strided columns
-    // (VAR_4A input, both outputs) run lanes along c, so that neighbouring
-    // lanes touch neighbouring addresses.
+    // (VAR_COL and VAR_4A input, every output) run lanes along c, so that
+    // neighbouring lanes touch neighbouring addresses.
     const int c = tid >> lgt, t = tid & (TPT - 1);
     const int c2 = tid & (G - 1), t2 = tid >> lgg;
     const long long b0 = (long long)blockIdx.x * G;

@@ -560,15 +560,16 @@ static bool gen_single_fits(int prec, int64_t n, bool rows) {
     return n <= gen_cap(prec) || (rows && n <= gen_row_cap(prec));
 }
 
-// With ipg, both four-step passes run in place in one LDS buffer
-// (k_gen_ipg) when they hold at most 16 points per thread: half the LDS of
-// the two-buffer kernel, so twice the blocks per CU.  Measured on 512 MiB
-// batches: complex64 N=100000 (250 x 400) 0.990 -> 0.831 ms, N=250000 (500
-// x 500) 1.012 -> 0.829 ms, complex128 N=100000 0.952 -> 0.700 ms.  The
-// strided axes of N-D plans stay on the two-buffer kernel: their long
-// columns leave 1-2 columns per block, and the in-place form's separate
-// store phase lost 2-7 % there (complex64 1200 x 1500 1.322 -> 1.345 ms,
-// complex128 0.752 -> 0.803 ms).
+// With ipg, column layouts (both four-step passes, strided axes of N-D
+// plans) run in place in one LDS buffer (k_gen_ipg) when they hold at most
+// 16 points per thread: half the LDS of the two-buffer kernel, so twice the
+// blocks per CU.  Measured on 512 MiB batches: complex64 N=100000 (250 x
+// 400) 0.990 -> 0.831 ms, N=250000 (500 x 500) 1.012 -> 0.829 ms,
+// complex128 N=100000 0.952 -> 0.700 ms.  Strided axes gain once blocks
+// grow to 512 threads (below): complex64 1200 x 1500 1.325 -> 0.812 ms,
+// 1000 x 1000 0.791 -> 0.570 ms, 250 x 250 x 64 1.179 -> 0.860 ms;
+// complex128 1200 x 1500 0.752 -> 0.656 ms, 1000 x 1000 0.755 -> 0.618 ms
+// (at 256 threads the in-place form had lost 2-7 % on them).
 static GenCfg gen_cfg(int64_t n, int prec, bool rows, bool ipg = false) {
     if (rows && gen_use_ip(n, prec)) {
         GenCfg c;
@@ -690,9 +691,14 @@ static int emit_gen_single(Plan& p, int di, const AxisJob& j, std::string& algo)
         return set_error(AFFT_ERR_NOT_IMPLEMENTED, "mixed radix: length too large for one pass on this layout");
     GArgs g{};
     int tpt, lds, threads, rc;
-    if ((rc = gen_args(p, j.n, var == VAR_ROW, g, &tpt, &lds, &threads))) return rc;
+    bool ipg = false;
+    // strided columns above 128 points (16 threads or more a column) on the
+    // in-place kernel; below, its separate store phase costs more than the
+    // LDS it saves (the 3-point axis of 3 x 5000 x 7: 2.7 % slower)
+    if ((rc = gen_args(p, j.n, var == VAR_ROW, g, &tpt, &lds, &threads, var == VAR_COL && j.n > 128 ? &ipg : nullptr)))
+        return rc;
     const bool ip = var == VAR_ROW && gen_use_ip(j.n, prec);
-    const void* fn = ip ? gen_ip_kernel(prec, dir) : gen_kernel(prec, var, dir);
+    const void* fn = ip ? gen_ip_kernel(prec, dir) : ipg ? gen_ipg_kernel(prec, var, dir) : gen_kernel(prec, var, dir);
     if ((rc = prep_dynamic_lds(fn, lds))) return rc;
     g.k.count = count;
     g.k.in_cd = j.in_cd;

@@ -181,3 +181,30 @@ def test_mixed_fourstep_passes(ctx, prec, n, split):
     print(f"n={n} split={split} algo={algo} relL2={err:.2e}")
     assert err <= tol, (n, split, err, algo)
     assert rel_l2(back, O.fft_batch(got.astype(np.complex128), inverse=True)) <= tol, (n, split, algo)
+
+
+# Strided smooth axes of N-D plans above 128 points run on the in-place
+# column kernel (below, on the two-buffer one): 16 to 256 threads a column,
+# transform counts that are not a multiple of the columns per block, an
+# axis with no unit-stride neighbour (the middle axis of a 3-D shape), both
+# directions, against the oracle's PlanND restatement.
+ND_SMOOTH = [(1200, 15), (300, 7, 5), (5, 1000, 3), (2187, 4), (3125, 3), (6, 250, 250)]
+
+
+@pytest.mark.parametrize("dims", ND_SMOOTH)
+@pytest.mark.parametrize("prec", [A.COMPLEX64, A.COMPLEX128])
+def test_strided_smooth_axes(ctx, dims, prec):
+    dt = np.complex64 if prec == A.COMPLEX64 else np.complex128
+    x = synth.uniform_complex(23 + len(dims), dims, dt)
+    with A.Plan.from_desc(ctx, dims, precision=prec) as p:
+        algo = p.describe()
+        got = np.empty_like(x)
+        p.forward(got, x)
+        back = np.empty_like(x)
+        p.inverse(back, got)
+    n = int(np.prod(dims))
+    tol = c64_tol(n) if prec == A.COMPLEX64 else C128_TOL
+    err = rel_l2(got, O.fftn(x.astype(np.complex128)))
+    print(f"dims={dims} algo={algo} relL2={err:.2e}")
+    assert err <= tol, (dims, err, algo)
+    assert rel_l2(back, O.fftn(got.astype(np.complex128), inverse=True)) <= tol, (dims, algo)
