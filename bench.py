@@ -133,7 +133,8 @@ def sweep_leg(torch, A, ctx, stream, steps, x4096=None):
 def smooth_leg(torch, A, ctx, stream, steps):
     """2/3/5/7-smooth lengths (the mixed-radix path) at batch 2^26/N, 512 MiB
     per buffer: N = 1000, 3000, 6000 and 12000 in one pass (6000 and 12000
-    with every pass in place in one LDS buffer), 100000 as four-step."""
+    with every pass in place in one LDS buffer), 100000 as four-step, and the
+    2-D 1000 x 1000 (rows, then the strided column axis)."""
     res = []
     g = torch.Generator(device="cuda:0").manual_seed(17)
     for n in (1000, 3000, 6000, 12000, 100000):
@@ -151,6 +152,23 @@ def smooth_leg(torch, A, ctx, stream, steps):
                     "rel_l2_vs_fp64": rel, "max_err_vs_fp64": mx})
         plan.close()
         del x, y
+    d = 1000
+    batch = (1 << 26) // (d * d)
+    x = torch.complex(torch.rand(batch * d * d, device="cuda:0", generator=g) * 2 - 1,
+                      torch.rand(batch * d * d, device="cuda:0", generator=g) * 2 - 1)
+    y = torch.empty_like(x)
+    plan = A.Plan.from_desc(ctx, (d, d), batch=batch)
+    ms = _time_plan(torch, plan, A, stream, y, x, steps)
+    sel = [0, batch - 1]
+    ref = np.fft.fft2(x.view(batch, d, d)[sel].cpu().numpy().astype(np.complex128))
+    got = y.view(batch, d, d)[sel].cpu().numpy().astype(np.complex128)
+    gbs = 16.0 * d * d * batch / (ms * 1e-3) / 1e9
+    res.append({"n": [d, d], "batch": batch, "algorithm": plan.describe(), "ms": round(ms, 5),
+                "gbs_algorithmic": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                "rel_l2_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+                "max_err_vs_fp64": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))})
+    plan.close()
+    del x, y
     torch.cuda.empty_cache()
     return res
 
