@@ -112,6 +112,60 @@ TUNED_SPLIT = {(20, COMPLEX64): 9, (23, COMPLEX64): 12, (24, COMPLEX64): 13,
                (18, COMPLEX128): 10, (19, COMPLEX128): 8, (23, COMPLEX128): 12}
 
 
+def _smooth_radices(m):
+    """plan.cpp smooth_schedule: odd radices first, then the power-of-two
+    remainder radix and radix-16 passes (powers of two: remainder first)."""
+    a = 0
+    while m % 2 == 0:
+        m //= 2
+        a += 1
+    odd = []
+    for q in (3, 5, 7):
+        while m % q == 0:
+            m //= q
+            odd.append(q)
+    if m != 1:
+        return []
+    tail = ([1 << (a % 4)] if a % 4 else []) + [16] * (a // 4)
+    return odd + tail
+
+
+def fourstep_split(n, prec):
+    """plan.cpp gen_fourstep_split: the default n1 of a smooth four-step."""
+    maxp = 16384 if prec == COMPLEX64 else 8192
+    gcap = 8192 if prec == COMPLEX64 else 4096
+    rcap = min(gcap, 4096)
+
+    def pow2(m):
+        return m & (m - 1) == 0
+
+    def usable(m):
+        return m <= maxp if pow2(m) else m <= rcap
+
+    def cost(m):
+        if pow2(m):
+            return 3.0
+        r = _smooth_radices(m)
+        return len(r) + r.count(2) + 0.5 * r.count(4)
+
+    rt = math.sqrt(n)
+    n1, best, bal = 0, 0.0, 0.0
+    d = 8
+    while d * 8 <= n:
+        if n % d == 0 and 0.2 * rt <= d <= 5.0 * rt and usable(d) and usable(n // d):
+            c, b = cost(d) + cost(n // d), abs(math.log(d / rt))
+            if not n1 or c < best - 1e-9 or (c < best + 1e-9 and b < bal - 1e-9):
+                n1, best, bal = d, c, b
+        d += 1
+    if not n1:
+        q = 1
+        while q * q <= n:
+            if n % q == 0 and q <= gcap and n // q <= gcap:
+                n1 = q
+            q += 1
+    return n1
+
+
 def heuristic_choice(shape, cands):
     """The built-in routing (plan.cpp emit_axis) expressed as a candidate."""
     n, prec = shape["n"], shape["precision"]
@@ -129,17 +183,7 @@ def heuristic_choice(shape, cands):
     elif smooth and n <= rcap:
         want = (STRATEGY_MIXED, 0)
     elif smooth:
-        p2 = n & -n
-        if 16 <= p2 <= 4096 and n // p2 <= 2048:
-            want = (STRATEGY_SIXSTEP, p2)   # compile-time column pass
-        else:
-            best = None  # most balanced divisor
-            q = 1
-            while q * q <= n:
-                if n % q == 0 and q <= gcap and n // q <= gcap:
-                    best = q
-                q += 1
-            want = (STRATEGY_SIXSTEP, best)
+        want = (STRATEGY_SIXSTEP, fourstep_split(n, prec))
     else:
         want = (STRATEGY_BLUESTEIN, 0)
     return want if want in cands else cands[0]

@@ -144,6 +144,7 @@ int split_twiddles(Plan& p, int prec, int64_t L, int* shift, const void** lo, co
 int max_single_pass(int prec);
 int gen_cap(int prec);
 int gen_row_cap(int prec);          // contiguous rows: one in-place pass
+int64_t gen_fourstep_split(int64_t n, int prec);   // default n1 of a smooth four-step
 
 // ---- planning knowledge (tune.cpp, mlp.hip) --------------------------------------
 constexpr int NFEAT = 32;

@@ -720,6 +720,49 @@ static int emit_gen_single(Plan& p, int di, const AxisJob& j, std::string& algo)
     return AFFT_OK;
 }
 
+// Default split n = n1 x n2 of a smooth four-step: the divisor pair of least
+// estimated cost, each factor a power of two held in LDS (the compile-time
+// kernel) or a runtime-radix length of 8 to 4096 points (the in-place
+// column kernel, at most 16 points per thread), with n1/sqrt(n) in
+// [0.2, 5].  A factor costs 3 on the compile-time kernel, else one per pass
+// plus one per radix-2 and a half per radix-4 pass; ties go to the more
+// balanced pair, then the smaller n1.  Fitted on 33 complex64 / complex128
+// lengths from 9000 to 10^6 with every such split timed on 512 MiB batches:
+// 5.4 % mean regret, 29 % at worst (5.8 % and 6.0 % on either half fitted
+// on the other), against 15.6 % and 71 % for the earlier rule (the
+// power-of-two part as the column
+// factor when 16..4096 points with the rest at most 2048, else the most
+// balanced divisors): complex64 18000 as 16 x 1125 took 0.99 ms against
+// 0.59 ms as 225 x 80, 30000 as 16 x 1875 1.06 against 0.62 ms.  Lengths
+// with no such pair take the most balanced divisors within the caps.
+int64_t gen_fourstep_split(int64_t n, int prec) {
+    const int64_t cap = gen_cap(prec), rcap = std::min<int64_t>(cap, 4096);
+    auto usable = [&](int64_t m) { return is_pow2(m) ? m <= max_single_pass(prec) : m <= rcap; };
+    auto cost = [&](int64_t m) {
+        if (is_pow2(m)) return 3.0;
+        const std::vector<int> r = smooth_schedule(m);
+        double c = (double)r.size();
+        for (int q : r) c += q == 2 ? 1.0 : q == 4 ? 0.5 : 0.0;
+        return c;
+    };
+    const double rt = std::sqrt((double)n);
+    int64_t n1 = 0;
+    double best = 0, bal = 0;
+    for (int64_t d = 8; d * 8 <= n; ++d) {
+        if (n % d || d < 0.2 * rt || d > 5.0 * rt || !usable(d) || !usable(n / d)) continue;
+        const double c = cost(d) + cost(n / d), b = std::fabs(std::log(d / rt));
+        if (!n1 || c < best - 1e-9 || (c < best + 1e-9 && b < bal - 1e-9)) {
+            n1 = d;
+            best = c;
+            bal = b;
+        }
+    }
+    if (!n1)
+        for (int64_t d = 1; d * d <= n; ++d)
+            if (n % d == 0 && d <= cap && n / d <= cap) n1 = d;
+    return n1;
+}
+
 // Four-step over runtime kernels for smooth n beyond one LDS pass.
 static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& algo) {
     const int prec = p.desc.precision;
@@ -727,19 +770,7 @@ static int emit_gen_fourstep(Plan& p, int di, const AxisJob& j, std::string& alg
     const int es = prec ? 16 : 8;
     if (j.in_nd != 1 || j.out_nd != 1) return set_error(AFFT_ERR_NOT_IMPLEMENTED, "four-step needs unit stride");
     const int cap = gen_cap(prec);
-    int64_t n1 = 0;
-    for (int64_t d = 1; d * d <= j.n; ++d)
-        if (j.n % d == 0 && d <= cap && j.n / d <= cap) n1 = d;
-    // The power-of-two part of n as the column factor, when it is 16 .. 4096
-    // points and the rest at most 2048: the column pass then runs on the
-    // compile-time kernel.  Over the 675 smooth four-step shapes of the
-    // tuning records this rule has 8.9 % mean regret against 12.3 % for the
-    // balanced split alone; N=12000 as 32 x 375 takes 0.62 ms against
-    // 0.76 ms as 100 x 120 on 512 MiB.  With a longer rest (100000 =
-    // 32 x 3125: 2.8 ms against 1.05 ms) or a long power-of-two column
-    // (344064 = 16384 x 21) it loses.
-    const int64_t p2 = j.n & -j.n;
-    if (p2 >= 16 && p2 <= 4096 && j.n / p2 <= 2048) n1 = p2;
+    int64_t n1 = gen_fourstep_split(j.n, prec);
     if (j.split) {
         // a power-of-two factor may use the compile-time kernels' larger cap
         auto fits = [&](int64_t f) { return f <= (is_pow2(f) ? (int64_t)max_single_pass(prec) : (int64_t)cap); };

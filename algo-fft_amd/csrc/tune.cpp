@@ -50,15 +50,21 @@ std::vector<Candidate> plan_candidates(const afft_plan_desc& d) {
         });
         for (size_t i = 0; i < divs.size() && i < 3; ++i) out.push_back({AFFT_STRATEGY_SIXSTEP, divs[i]});
         // the splits whose column or row factor is the power-of-two part of
-        // n: that pass runs on the compile-time Stockham kernel
+        // n (that pass runs on the compile-time Stockham kernel), and the
+        // built-in routing's default split
+        auto add = [&](int64_t s) {
+            bool seen = false;
+            for (const Candidate& c : out) seen |= c.strategy == AFFT_STRATEGY_SIXSTEP && c.split == s;
+            if (!seen) out.push_back({AFFT_STRATEGY_SIXSTEP, s});
+        };
         const int64_t p2 = n & -n;
         for (int64_t s : {p2, n / p2}) {
             const int64_t o = n / s;
             if (p2 < 16 || s > (s == p2 ? maxp : gcap) || o > (o == p2 ? maxp : gcap)) continue;
-            bool seen = false;
-            for (const Candidate& c : out) seen |= c.strategy == AFFT_STRATEGY_SIXSTEP && c.split == s;
-            if (!seen) out.push_back({AFFT_STRATEGY_SIXSTEP, s});
+            add(s);
         }
+        const int64_t s0 = gen_fourstep_split(n, prec);
+        if (s0 > 1 && s0 < n) add(s0);
     }
     // Bluestein is the only route for other sizes; for smooth sizes it is a
     // candidate while its padded length stays modest

@@ -208,3 +208,18 @@ def test_strided_smooth_axes(ctx, dims, prec):
     print(f"dims={dims} algo={algo} relL2={err:.2e}")
     assert err <= tol, (dims, err, algo)
     assert rel_l2(back, O.fftn(got.astype(np.complex128), inverse=True)) <= tol, (dims, algo)
+
+
+@pytest.mark.parametrize("prec,n", [(A.COMPLEX64, 18000), (A.COMPLEX64, 24000), (A.COMPLEX64, 100000),
+                                    (A.COMPLEX128, 9000), (A.COMPLEX128, 36000)])
+def test_default_smooth_fourstep_split(ctx, prec, n):
+    # the built-in routing's split equals its Python mirror (the planner's
+    # heuristic baseline) and the plan matches the oracle
+    from algofft_hip import planner
+    dt = np.complex64 if prec == A.COMPLEX64 else np.complex128
+    x = synth.uniform_complex(31 + n, (2, n), dt)
+    got, algo = _plan_run(ctx, x)
+    s = planner.fourstep_split(n, prec)
+    assert algo == f"hip_mixed4step{s}x{n // s}", (algo, s)
+    tol = c64_tol(n) if prec == A.COMPLEX64 else C128_TOL
+    assert rel_l2(got, O.fft_batch(x.astype(np.complex128))) <= tol, algo

@@ -56,3 +56,19 @@ def test_oracle_mlp_gradient_by_finite_differences():
         fd = (O.mlp_loss_grad(Pp, X, T)[0] - O.mlp_loss_grad(Pm, X, T)[0]) / (2 * h)
         assert abs(fd - G[idx]) <= 1e-6 + 1e-5 * abs(fd), (idx, fd, G[idx])
     assert math.isfinite(loss)
+
+
+def test_default_smooth_fourstep_split_is_a_candidate():
+    # the built-in routing's split (plan.cpp gen_fourstep_split, mirrored by
+    # planner.fourstep_split) is among the candidates the tuner and the cost
+    # model rank; picks checked against the timed sweep it was fitted on
+    want = {(A.COMPLEX64, 18000): 125, (A.COMPLEX64, 24000): 120, (A.COMPLEX64, 100000): 200,
+            (A.COMPLEX64, 1000000): 1000, (A.COMPLEX128, 9000): 75, (A.COMPLEX128, 36000): 180}
+    for (prec, n), s in want.items():
+        assert planner.fourstep_split(n, prec) == s, (n, prec)
+    for prec in (A.COMPLEX64, A.COMPLEX128):
+        for n in (18000, 20000, 36000, 72000, 100000, 250000, 480000, 1000000, 344064):
+            s = planner.fourstep_split(n, prec)
+            assert n % s == 0
+            c = planner.candidates({"n": n, "batch": 4, "precision": prec, "stride": 1})
+            assert (A.STRATEGY_SIXSTEP, s) in c, (n, prec, s, c)
