@@ -150,13 +150,11 @@ def fourstep_split(n, prec):
 
     rt = math.sqrt(n)
     n1, best, bal = 0, 0.0, 0.0
-    d = 8
-    while d * 8 <= n:
-        if n % d == 0 and 0.2 * rt <= d <= 5.0 * rt and usable(d) and usable(n // d):
+    for d in range(max(8, math.ceil(0.2 * rt)), min(n // 8, int(5.0 * rt)) + 1):
+        if n % d == 0 and usable(d) and usable(n // d):
             c, b = cost(d) + cost(n // d), abs(math.log(d / rt))
             if not n1 or c < best - 1e-9 or (c < best + 1e-9 and b < bal - 1e-9):
                 n1, best, bal = d, c, b
-        d += 1
     if not n1:
         q = 1
         while q * q <= n:

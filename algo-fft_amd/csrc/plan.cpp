@@ -748,8 +748,9 @@ int64_t gen_fourstep_split(int64_t n, int prec) {
     const double rt = std::sqrt((double)n);
     int64_t n1 = 0;
     double best = 0, bal = 0;
-    for (int64_t d = 8; d * 8 <= n; ++d) {
-        if (n % d || d < 0.2 * rt || d > 5.0 * rt || !usable(d) || !usable(n / d)) continue;
+    const int64_t lo = std::max<int64_t>(8, (int64_t)std::ceil(0.2 * rt)), hi = std::min<int64_t>(n / 8, (int64_t)(5.0 * rt));
+    for (int64_t d = lo; d <= hi; ++d) {
+        if (n % d || !usable(d) || !usable(n / d)) continue;
         const double c = cost(d) + cost(n / d), b = std::fabs(std::log(d / rt));
         if (!n1 || c < best - 1e-9 || (c < best + 1e-9 && b < bal - 1e-9)) {
             n1 = d;
